@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Headline benchmark: link updates/s at 48^4, FP32 bulk vs FP64 plain, on one GPU.
+
+Workload (one "step" = one sweep of the hot path over the whole lattice):
+  48^4 compact U(1), Wilson action, beta = 1.0, hot start;
+  sweep = 1 Metropolis pass (4 hits, theta' = theta + U[-0.5, 0.5)) + 3
+  overrelaxation passes, each pass = 8 checkerboard (parity, mu) launches;
+  the plaquette is measured after every sweep (FP64 reduction, on the device).
+Defaults follow the 48^4 protocol: 500 thermalisation sweeps (--warmup) then
+1000 measured, timed sweeps (--steps), for the FP32-bulk path and again for
+the FP64 path on the same GPU.  A link update is one link visited by one pass,
+so a sweep is 16 V = 84,934,656 link updates.
+
+The JSON line carries:
+  value          FP32-bulk link updates/s (whole timed region, inputs resident)
+  roofline       dominant FP32 phase kernel: algorithmic bytes per launch
+                 (9 * 4 B per link update * V/2 links) / mean launch duration
+                 from HIP events on the library stream, against 8 TB/s HBM
+  cpu_baseline   the FP64 host oracle (OpenMP) on a bounded sample of the
+                 same sweeps, timed on this host's cores
+  fp64 / ratio / plaquette errors as extra fields.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def jackknife(x, nbins=20):
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x) // nbins * nbins
+    if n < nbins:
+        return float(np.mean(x)), float("nan")
+    b = x[:n].reshape(nbins, -1).mean(axis=1)
+    tot = b.sum()
+    jk = (tot - b) / (nbins - 1)
+    mean = jk.mean()
+    err = math.sqrt((nbins - 1) / nbins * np.sum((jk - mean) ** 2))
+    return float(x[:n].mean()), err
+
+
+def run_path(u1, prec, dims, beta, seed, warmup, steps):
+    g = u1.geom(dims, beta, 1.0)
+    lat = u1.Lattice.cold(g, prec)
+    lat.hotstart(seed)
+    m = u1.BASELINE_METRO
+    if warmup > 0:
+        lat.bench(1, 3, m, seed, 0, warmup, False)  # thermalisation (also runs the autotuner)
+    ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, True)
+    return lat, ms, kernel_ms, series
+
+
+def cpu_baseline(dims, beta, seed, budget_s):
+    """FP64 host oracle (the timed CPU baseline), bounded to about budget_s."""
+    from oracle.oracle import Oracle
+
+    orc = Oracle()
+    og = orc.geom(dims, beta, 1.0)
+    lat = orc.hotstart(og, seed)
+    om = orc.metro(1, 4, 0.5)
+    V = int(np.prod(dims))
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.update(og, lat, 1, 3, om, seed, n)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 5 or el * (n + 1) / n > 3 * budget_s:
+            break
+    return {"value": 16.0 * V * n / el, "unit": "link updates/s", "cores": orc.threads(), "kind": "port",
+            "sample": f"{n} sweep(s) of the same 48^4 workload (FP64 OpenMP oracle, hot start), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--dims", type=str, default="48,48,48,48")
+    ap.add_argument("--beta", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=20241)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding along t is described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+    dims = tuple(int(v) for v in args.dims.split(","))
+    V = int(np.prod(dims))
+    lu_per_sweep = 16.0 * V
+
+    import latticeu1_amd as u1
+
+    u1.init(0)
+    dev = u1.device_name()
+
+    res = {}
+    for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
+        if prec == u1.F64 and args.skip_fp64:
+            continue
+        print(f"[bench] {name}: {args.warmup} thermalisation + {args.steps} timed sweeps at {dims}", file=sys.stderr,
+              flush=True)
+        lat, ms, kms, series = run_path(u1, prec, dims, args.beta, args.seed, args.warmup, args.steps)
+        pm = np.array([u1.mean_plaquette(r) for r in series])
+        mean, err = jackknife(pm)
+        s = 4 if prec == u1.F32 else 8
+        bytes_per_launch = 4.5 * V * s  # 4V angles read + V/2 written per (parity, mu) phase
+        res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, plaq=mean,
+                         plaq_err=err, bytes=bytes_per_launch, elem=s)
+        print(f"[bench] {name}: {res[name]['rate']:.4e} link updates/s, {ms / args.steps:.3f} ms/sweep, "
+              f"<P> = {mean:.8f} +- {err:.1e}, mean phase kernel ms M/OR = {kms[0]:.4f}/{kms[1]:.4f}",
+              file=sys.stderr, flush=True)
+
+    f32 = res["fp32"]
+    # dominant kernel class: the one with the larger total time per sweep
+    t_m, t_o = 8 * f32["kms"][0], 24 * f32["kms"][1]
+    dom = "overrelaxation" if t_o >= t_m else "metropolis"
+    dom_ms = f32["kms"][1] if dom == "overrelaxation" else f32["kms"][0]
+    achieved = f32["bytes"] / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"k_phase<float> {dom} (parity, mu) phase",
+                "bytes_per_launch": f32["bytes"], "mean_launch_ms": round(dom_ms, 5),
+                "other_kernel": {"metropolis_ms": round(f32["kms"][0], 5), "overrelaxation_ms": round(f32["kms"][1], 5)},
+                "sweep_algorithmic_GBs": round(32 * f32["bytes"] / ((t_m + t_o) * 1e-3) / 1e9, 1)}
+
+    extra = {}
+    if "fp64" in res:
+        f64 = res["fp64"]
+        extra["fp64_link_updates_per_s"] = f64["rate"]
+        extra["fp32_over_fp64"] = f32["rate"] / f64["rate"]
+        extra["fp64_kernel_ms"] = {"metropolis": round(f64["kms"][0], 5), "overrelaxation": round(f64["kms"][1], 5)}
+        d = f32["plaq"] - f64["plaq"]
+        comb = math.hypot(f32["plaq_err"], f64["plaq_err"])
+        extra["plaquette"] = {"fp32": [f32["plaq"], f32["plaq_err"]], "fp64": [f64["plaq"], f64["plaq_err"]],
+                              "diff": d, "diff_over_combined_err": d / comb if comb > 0 else None,
+                              "within_1sigma_of_fp64": abs(d) <= f64["plaq_err"]}
+        # fixed configuration: FP32 measurement of the final FP64 configuration
+        p64 = f64["lat"].plaquette()
+        p32 = f64["lat"].astype(u1.F32).plaquette()
+        extra["fixed_config_plaquette_rel_err"] = float(abs(u1.mean_plaquette(p32) - u1.mean_plaquette(p64))
+                                                        / abs(u1.mean_plaquette(p64)))
+
+    cpu = None
+    if not args.skip_cpu:
+        print(f"[bench] cpu baseline (oracle, ~{args.cpu_budget:.0f} s)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(dims, args.beta, args.seed, args.cpu_budget)
+
+    out = {
+        "metric": "link updates/s at 48^4, FP32-bulk vs FP64 same GPU; % HBM roofline; <P> err",
+        "value": f32["rate"],
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (hot-start gauge field from Philox seed)",
+        "config": {"workload": "U(1) Wilson gauge, 48^4, beta=1.0, hot start; sweep = 1 Metropolis "
+                               "(4 hits, delta U[-0.5,0.5)) + 3 overrelaxation; plaquette every sweep",
+                   "lattice": "x".join(str(d) for d in dims), "precision": "fp32 angles, fp64 reductions",
+                   "link_updates_per_sweep": lu_per_sweep, "device": dev},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

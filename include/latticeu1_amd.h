@@ -1,0 +1,199 @@
+/* latticeu1_amd.h -- C ABI of the MI355X (gfx950) LatticeU1 hot path.
+ *
+ * Plain pointers, sizes and status codes; no HIP, torch or C++ types.  Every
+ * function returns U1_OK (0) or an error code; u1_last_error() describes the
+ * last failure of the calling thread.  Device pointers are allocated with
+ * u1_malloc (zero filled, as the reference's dev_malloc, alloc.cpp:193) and all
+ * work runs in order on one library stream.  Functions that hand results to
+ * host memory synchronise; updates are asynchronous until the next
+ * synchronising call (u1_sync, a measurement, a copy to the host).
+ *
+ * Lattice layout (reference index.h:124-132): 4V angles theta_mu(x) in
+ * even/odd structure-of-arrays order lat[eo + parity*V/2 + mu*V],
+ * eo = lex(x) >> 1, lex = x0 + Nx*(x1 + Ny*(x2 + Nz*x3)); angles in [-pi, pi).
+ * prec selects binary64 (U1_F64, "plain") or binary32 (U1_F32, "FP32 bulk")
+ * angles; all reductions are binary64 either way.  4D lattices only.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   u1_init / u1_finalize        Start / Finalize            include/cuda_error_check.h:45-48
+ *   u1_malloc/u1_free/u1_memcpy  dev_malloc, Array::Copy     include/alloc.h:60-67, include/array.h:60-82
+ *   u1_hotstart                  HotStart                    include/update.h:10
+ *   u1_update                    UpdateLattice               include/update.h:12
+ *   u1_plaquette                 Plaquette                   include/plaquette.h:12
+ *   u1_polyakov                  Polyakov                    include/polyakov.h:13
+ *   u1_poly_volume, u1_pp,       Poly2 (Polyakov_Vol + PP)   include/polyakov.h:16, src/polyakov.cu:618-654
+ *   u1_poly2
+ *   u1_multilevel                MultiLevel / MultiLevelField include/multilevel.h:9,25
+ *   u1_convert_layout            LatticeConvert              include/array.h:186
+ *   u1_plaquette_fields          PlaquetteFields             include/plaquette.h:34
+ *   u1_tune_*                    setTuning, tune cache       include/parameters.h:33-37, src/tune.cpp:160-295
+ *   u1_ensemble_*                (no counterpart: the reference holds one
+ *                                 lattice per process in __constant__ memory,
+ *                                 src/parameters.cu:78-88)
+ */
+#ifndef LATTICEU1_AMD_H
+#define LATTICEU1_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  U1_OK = 0,
+  U1_ERR_ARG = 1,         /* invalid argument (odd extent, null pointer, ...) */
+  U1_ERR_HIP = 2,         /* HIP runtime error */
+  U1_ERR_STATE = 3,       /* library not initialised */
+  U1_ERR_UNSUPPORTED = 4, /* valid in the reference but not built here */
+  U1_ERR_IO = 5           /* file I/O (tune cache) */
+};
+
+enum { U1_F64 = 0, U1_F32 = 1 };
+enum { U1_PROPOSAL_INDEPENDENT = 0, U1_PROPOSAL_DELTA = 1 };
+enum { U1_POLY_PLAIN = 0, U1_POLY_MULTIHIT = 1, U1_POLY_MULTIHIT_STOCHASTIC = 2 };
+enum { U1_H2H = 0, U1_H2D = 1, U1_D2H = 2, U1_D2D = 3 };
+/* Philox stream tags (counter word 2, bits 24-31) */
+enum {
+  U1_STREAM_UPDATE = 0,
+  U1_STREAM_HOTSTART = 1,
+  U1_STREAM_MULTIHIT = 2,
+  U1_STREAM_ML_L4 = 3,
+  U1_STREAM_ML_L2 = 4,
+  U1_STREAM_ML_MULTIHIT = 5
+};
+
+/* Geometry and couplings (reference SetupLatticeParameters, parameters.cu:146-208;
+ * dims = Grid[0..3] = Nx, Ny, Nz, Nt, all even). */
+typedef struct {
+  int32_t dims[4];
+  double beta;
+  double aniso; /* xi: space-space staples weighted 1/xi, space-time xi (staple.h:75-76) */
+} u1_geom;
+
+/* Metropolis settings.  Reference behaviour: proposal INDEPENDENT, nhit 1
+ * (update.cu:181-182).  BASELINE-style runs: DELTA, nhit 4, step 0.5. */
+typedef struct {
+  int32_t proposal;
+  int32_t nhit;
+  double step; /* delta = step * U[-1, 1) */
+} u1_metro;
+
+/* ---------------------------------------------------------------- runtime */
+int u1_init(int device);
+int u1_finalize(void);
+int u1_sync(void);
+const char* u1_last_error(void);
+int u1_device_name(char* buf, size_t len);
+
+/* ---------------------------------------------------------------- memory */
+int u1_malloc(void** ptr, size_t bytes);
+int u1_free(void* ptr);
+int u1_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int u1_zero(void* ptr, size_t bytes);
+/* n elements, FP64 <-> FP32 (or same-precision copy) on the device */
+int u1_convert_precision(const void* src, int src_prec, void* dst, int dst_prec, int64_t n);
+
+/* ---------------------------------------------------------------- updates */
+/* Hot start: theta = pi*(2u - 1), u from Philox stream HOTSTART, c0 = link index. */
+int u1_hotstart(const u1_geom* g, int prec, void* lat, uint64_t seed, uint32_t replica);
+/* One UpdateLattice(metrop, ovrn): metrop Metropolis passes then ovrn
+ * overrelaxation passes, each over parity 0,1 and mu 0..3.  Philox counter
+ * (link, sweep, stream|pass|hitpair, c3).  mask_ml 0, 2 or 4: multilevel
+ * sublattice mask (spatial links on slices t % mask_ml == 0 frozen).
+ * accepts (nullable, synchronises): accepted Metropolis hits are added. */
+int u1_update(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+              uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t c3, int32_t mask_ml, uint64_t* accepts);
+
+/* One checkerboard phase: a single launch of kernel_metropolis (kind 0, with
+ * Metropolis pass index `pass`) or kernel_overrelaxation (kind 1) on parity
+ * `parity`, direction mu (update.cu:177-190, 227-231). */
+int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity, int32_t mu, const u1_metro* m,
+             uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t pass, uint32_t c3, int32_t mask_ml);
+
+/* ---------------------------------------------------------------- measurements */
+/* out[4] = {SS.re, SS.im, ST.re, ST.im}: sum of 1 - e^{iP} over space-space and
+ * space-time plaquettes, normalised by 1/(3V) each (plaquette.cu:156-177).
+ * Mean plaquette <P> = 1 - (out[0] + out[2]) / 2. */
+int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]);
+/* out[2] = Polyakov loop (1/SV) sum_x e^{i sum_t theta_t(x,t)}. */
+int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]);
+/* Per spatial site P(x) into P_dev (2*SV doubles, lexicographic spatial index).
+ * mode U1_POLY_*; mh (stochastic mode only): proposal / nhit / step; Philox
+ * counter (link, c1, stream|0|hitpair, c3). */
+int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
+                   uint32_t stream, uint32_t c1, uint32_t c3, void* P_dev);
+/* C(R) = (1/(3 SV)) sum_x sum_dir P(x) conj P(x + R e_dir), R = 1..rmax, into out[2*rmax]. */
+int u1_pp(const u1_geom* g, const void* P_dev, int32_t rmax, double* out);
+/* Poly2: u1_poly_volume then u1_pp with rmax = Nx/2; out[2*(Nx/2)]. */
+int u1_poly2(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
+             uint32_t c1, uint32_t c3, double* out);
+/* Per-site temporal-link field (2*V doubles, lexicographic 4D index), multilevel.cu:239-258. */
+int u1_temporal_links(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh,
+                      uint64_t seed, uint32_t stream, uint32_t c1, uint32_t c3, void* out_dev);
+/* Two-level multilevel correlator (multilevel.cu:639-732) on a copy of lat.
+ * Sublattice updates: metrop/ovrn passes with m, Philox streams ML_L4 / ML_L2,
+ * c1 = step index, c3 = call_id.  Temporal links: mh_mode (U1_POLY_MULTIHIT as
+ * the reference, or U1_POLY_MULTIHIT_STOCHASTIC with mh).  out[2*(Nx/2)];
+ * ppspace_dev (nullable): 2*SV*(Nx/2)*3 doubles (MultiLevelField).  Nt % 4 == 0. */
+int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32_t k4, int32_t n2, int32_t k2,
+                  int32_t metrop, int32_t ovrn, const u1_metro* m, int32_t mh_mode, const u1_metro* mh,
+                  uint64_t seed, uint32_t call_id, double* out, void* ppspace_dev);
+/* Layout permutation EO <-> lexicographic (convertOrder.cu:32-63); out distinct from in. */
+int u1_convert_layout(const u1_geom* g, int prec, const void* in, void* out, int32_t eo_to_no);
+/* Six plaquette components per site (plaquette_comps.cu:34-107): field_dev gets
+ * 2*6*size doubles (size = V if spacetime else SV), means[12] the normalised means. */
+int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spacetime, int32_t eo_order,
+                        void* field_dev, double means[12]);
+
+/* ---------------------------------------------------------------- batched replicas */
+typedef struct u1_ensemble u1_ensemble;
+/* nrep independent lattices (own geometry, beta, xi, seed; hot[r] != 0 for a hot
+ * start, else cold) packed in one buffer of precision prec.  Replica r uses
+ * Philox c3 = r.  Every sweep (metrop Metropolis + ovrn OR passes) of all
+ * replicas is one launch per (pass, parity, mu), captured once in a hipGraph. */
+int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, const uint64_t* seeds,
+                       const int32_t* hot, int prec, int32_t metrop, int32_t ovrn, const u1_metro* m);
+int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph);
+/* out[4*nrep]: per-replica plaquette as u1_plaquette. */
+int u1_ensemble_plaquette(u1_ensemble* e, double* out);
+/* Replica r angles as FP64 host array (4V, EO layout); write sets them. */
+int u1_ensemble_read(u1_ensemble* e, int32_t r, double* host);
+int u1_ensemble_write(u1_ensemble* e, int32_t r, const double* host);
+int u1_ensemble_info(u1_ensemble* e, uint64_t* sweeps_done, int64_t* total_links);
+int u1_ensemble_destroy(u1_ensemble* e);
+
+/* ---------------------------------------------------------------- autotuner */
+/* Tuning picks the workgroup size per (kernel, geometry, precision) by timing
+ * candidates on a scratch copy (results are independent of the choice). */
+int u1_tune_enable(int32_t on);
+int u1_tune_load(const char* path);
+int u1_tune_save(const char* path);
+/* Number of tuned entries; if key/val are given, entry i as text. */
+int u1_tune_entries(int32_t* n);
+
+/* ---------------------------------------------------------------- timing helper */
+/* Run nsweeps of u1_update (sweep indices sweep0, sweep0+1, ...) and return the
+ * elapsed device time in ms measured with HIP events on the library stream, and
+ * the mean duration of each phase kernel class (Metropolis, OR) if kernel_ms != NULL
+ * (kernel_ms[0] = mean Metropolis phase, kernel_ms[1] = mean OR phase, measured
+ * in a separate event-bracketed pass over the same number of sweeps). */
+int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+                     uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
+                     double* elapsed_ms, double* kernel_ms);
+
+/* ---------------------------------------------------------------- self test */
+/* Device evaluation of the shared elementary functions (no reference
+ * counterpart; lets tests check bit identity with the host oracle).
+ * fn 0: sincos_d(x) -> out[2i], out[2i+1] = sin, cos;  1: exp_d(x);
+ *    2: atan2_d(x, y) -> atan2(x[i], y[i]);  3: sincos_f((float)x);
+ *    4: atan2_f((float)x, (float)y);  5: Bessel I0, I1 (out 2 per input);
+ *    6: wrap_d(x).  x, y, out are host arrays. */
+int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n);
+/* Philox4x32-10 on the device: ctr[4n], key[2n] -> out[4n] (host arrays). */
+int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,989 @@
+// u1_capi.hip -- implementation of the C ABI (include/latticeu1_amd.h):
+// runtime context, argument checks, launch geometry, the autotuner, the
+// multilevel driver and the batched-replica ensemble with its hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/latticeu1_amd.h"
+#include "u1_kernels.h"
+
+using u1d::PhaseArgs;
+using u1d::RepDesc;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Ctx {
+  bool init = false;
+  int device = 0;
+  hipStream_t st = nullptr;
+  double* partials = nullptr;  // reduction scratch
+  size_t partials_cap = 0;     // in doubles
+  unsigned long long* counter = nullptr;
+  double* dres = nullptr;  // small device result buffer (64 doubles)
+  bool tune = true;
+  std::map<std::string, int> tunecache;
+  int tune_loaded = 0;
+};
+Ctx G;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) return fail(U1_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define REQ_INIT() \
+  if (!G.init) return fail(U1_ERR_STATE, "u1_init has not been called")
+#define RET(expr)             \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != U1_OK) return rc_; \
+  } while (0)
+
+// parameters.cu:163-168 rejects odd extents; the kernels here also need
+// 4 * V < 2^31 (32-bit link index and Philox counter word).
+int check_geom(const u1_geom* g) {
+  if (!g) return fail(U1_ERR_ARG, "null geometry");
+  int64_t v = 1;
+  for (int i = 0; i < 4; ++i) {
+    if (g->dims[i] < 2 || (g->dims[i] % 2) != 0)
+      return fail(U1_ERR_ARG, "Number of points should be an even number >= 2 in every direction");
+    v *= g->dims[i];
+  }
+  if (4 * v >= (int64_t)1 << 31) return fail(U1_ERR_ARG, "lattice too large: 4*V must be < 2^31");
+  if (!(g->aniso > 0.0)) return fail(U1_ERR_ARG, "anisotropy must be > 0");
+  return U1_OK;
+}
+
+int check_prec(int prec) {
+  if (prec != U1_F64 && prec != U1_F32) return fail(U1_ERR_ARG, "precision must be U1_F64 or U1_F32");
+  return U1_OK;
+}
+
+int check_metro(const u1_metro* m) {
+  if (!m) return fail(U1_ERR_ARG, "null Metropolis settings");
+  if (m->proposal != U1_PROPOSAL_INDEPENDENT && m->proposal != U1_PROPOSAL_DELTA)
+    return fail(U1_ERR_ARG, "unknown proposal kind");
+  if (m->nhit < 1 || m->nhit > 8190) return fail(U1_ERR_ARG, "nhit must be in [1, 8190]");
+  return U1_OK;
+}
+
+RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
+  RepDesc r{};
+  for (int i = 0; i < 4; ++i) r.N[i] = g->dims[i];
+  r.nxh = r.N[0] / 2;
+  r.V = r.N[0] * r.N[1] * r.N[2] * r.N[3];
+  r.HV = r.V / 2;
+  r.SV = r.N[0] * r.N[1] * r.N[2];
+  r.sy = r.nxh;
+  r.sz = r.nxh * r.N[1];
+  r.st = r.nxh * r.N[1] * r.N[2];
+  r.blk_begin = 0;
+  r.nblk = 0;
+  r.c3 = c3;
+  const u1d::MagicDiv a = u1d::make_magic((uint32_t)r.nxh), b = u1d::make_magic((uint32_t)r.N[1]),
+                     c = u1d::make_magic((uint32_t)r.N[2]);
+  r.div_nxh_m = a.m;
+  r.div_nxh_s = a.s;
+  r.div_ny_m = b.m;
+  r.div_ny_s = b.s;
+  r.div_nz_m = c.m;
+  r.div_nz_s = c.s;
+  r.beta = g->beta;
+  r.aniso = g->aniso;
+  r.seed = seed;
+  r.lat_off = 0;
+  return r;
+}
+
+size_t elem_size(int prec) { return prec == U1_F64 ? sizeof(double) : sizeof(float); }
+
+int ensure_partials(size_t n) {
+  if (n <= G.partials_cap) return U1_OK;
+  if (G.partials) HIPCHK(hipFree(G.partials));
+  G.partials = nullptr;
+  size_t cap = std::max<size_t>(n, 1 << 16);
+  HIPCHK(hipMalloc((void**)&G.partials, cap * sizeof(double)));
+  G.partials_cap = cap;
+  return U1_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Own tuner (replacing the QUDA-derived tune.cpp:306-416): workgroup size per
+// (kernel class, geometry, precision), chosen by timing candidates with HIP
+// events on a scratch copy of the lattice, so tuning never perturbs the chain
+// (the reference backs up and restores lattice and RNG, tune.h preTune).
+const int kCandidates[] = {64, 128, 256};
+
+std::string vol_str(const u1_geom* g) {
+  std::ostringstream s;
+  s << g->dims[0] << "x" << g->dims[1] << "x" << g->dims[2] << "x" << g->dims[3];
+  return s.str();
+}
+std::string tune_key(const char* name, const u1_geom* g, int prec) {
+  return vol_str(g) + "\t" + name + (prec == U1_F64 ? "_f64" : "_f32");
+}
+
+int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, int block, float* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // warm-up
+  HIPCHK(u1d::launch_phase(prec, kind, 0, lat, a, nb, block, G.st));
+  HIPCHK(hipEventRecord(e0, G.st));
+  for (int rep = 0; rep < 2; ++rep)
+    for (int mu = 0; mu < 4; ++mu) HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, nb, block, G.st));
+  HIPCHK(hipEventRecord(e1, G.st));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  return U1_OK;
+}
+
+// Block size for the update phase kernel class `kind` on this geometry.
+int phase_block(int prec, int kind, const u1_geom* g, const void* lat, const PhaseArgs& proto, int* block) {
+  const char* name = kind == u1d::KIND_METROPOLIS ? "Metropolis" : "OverRelaxation";
+  const std::string key = tune_key(name, g, prec);
+  auto it = G.tunecache.find(key);
+  if (it != G.tunecache.end()) {
+    *block = it->second;
+    return U1_OK;
+  }
+  if (!G.tune) {
+    *block = 256;
+    return U1_OK;
+  }
+  const int64_t bytes = 4 * (int64_t)proto.single.V * (int64_t)elem_size(prec);
+  void* scratch = nullptr;
+  HIPCHK(hipMalloc(&scratch, bytes));
+  HIPCHK(hipMemcpyAsync(scratch, lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  float best = 1e30f;
+  int bestb = 256;
+  for (int b : kCandidates) {
+    PhaseArgs a = proto;
+    a.accepts = nullptr;
+    const int nb = (proto.single.HV + b - 1) / b;
+    float ms = 0;
+    int rc = time_phase(prec, kind, scratch, a, nb, b, &ms);
+    if (rc != U1_OK) {
+      (void)hipFree(scratch);
+      return rc;
+    }
+    if (ms < best) {
+      best = ms;
+      bestb = b;
+    }
+  }
+  HIPCHK(hipFree(scratch));
+  G.tunecache[key] = bestb;
+  *block = bestb;
+  return U1_OK;
+}
+
+// One UpdateLattice on a single lattice (update.cu:449-456 loop order).
+int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, const u1_metro* m, uint64_t seed,
+               uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
+               hipEvent_t* kev = nullptr, std::vector<std::pair<int, int>>* kev_kind = nullptr) {
+  PhaseArgs a{};
+  a.single = make_rep(g, seed, c3);
+  a.reps = nullptr;
+  a.blockmap = nullptr;
+  a.ml = ml;
+  a.proposal = m->proposal;
+  a.nhit = m->nhit;
+  a.step = m->step;
+  a.stream = stream;
+  a.sweep = sweep;
+  a.sweep_dev = nullptr;
+  a.accepts = acc_dev;
+  int bm = 256, bo = 256;
+  if (metrop > 0) RET(phase_block(prec, u1d::KIND_METROPOLIS, g, lat, a, &bm));
+  if (ovrn > 0) RET(phase_block(prec, u1d::KIND_OVERRELAX, g, lat, a, &bo));
+  const int nbm = (a.single.HV + bm - 1) / bm, nbo = (a.single.HV + bo - 1) / bo;
+  int ev = 0;
+  for (int p = 0; p < metrop; ++p)
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.pass = (uint32_t)p;
+        a.parity = parity;
+        if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
+        HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, nbm, bm, G.st));
+        if (kev) {
+          HIPCHK(hipEventRecord(kev[2 * ev + 1], G.st));
+          kev_kind->push_back({ev, 0});
+          ++ev;
+        }
+      }
+  for (int o = 0; o < ovrn; ++o)
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.pass = (uint32_t)o;
+        a.parity = parity;
+        if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
+        HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, nbo, bo, G.st));
+        if (kev) {
+          HIPCHK(hipEventRecord(kev[2 * ev + 1], G.st));
+          kev_kind->push_back({ev, 1});
+          ++ev;
+        }
+      }
+  return U1_OK;
+}
+
+// Plaquette into a device result slot dst[4] (no synchronisation).
+int plaquette_async(const u1_geom* g, int prec, const void* lat, double* dst) {
+  const RepDesc r = make_rep(g, 0, 0);
+  const int block = 256;
+  const int nblk = (r.V + block - 1) / block;
+  RET(ensure_partials(4 * (size_t)nblk));
+  HIPCHK(u1d::launch_plaquette(prec, lat, r, G.partials, nblk, block, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 4, 1.0 / (3.0 * (double)r.V), dst, G.st));
+  return U1_OK;
+}
+
+int copy_back(double* host, const double* dev, size_t n) {
+  HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  return U1_OK;
+}
+
+}  // namespace
+
+struct u1_ensemble {
+  int32_t nrep = 0;
+  int prec = U1_F32;
+  int32_t metrop = 1, ovrn = 3;
+  u1_metro m{};
+  std::vector<u1_geom> geoms;
+  std::vector<RepDesc> upd_host, meas_host;  // per-replica descriptors (update / measurement launches)
+  RepDesc* upd_dev = nullptr;
+  RepDesc* meas_dev = nullptr;
+  uint16_t* upd_map = nullptr;
+  uint16_t* meas_map = nullptr;
+  int upd_blocks = 0, meas_blocks = 0;
+  void* lat = nullptr;
+  int64_t total_links = 0;
+  uint32_t* sweep_dev = nullptr;
+  uint64_t sweeps_done = 0;
+  double* res_dev = nullptr;
+  double* partials = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+const int kEnsBlock = 256;
+
+int ensemble_enqueue_sweep(u1_ensemble* e) {
+  PhaseArgs a{};
+  a.reps = e->upd_dev;
+  a.blockmap = e->upd_map;
+  a.ml = 0;
+  a.proposal = e->m.proposal;
+  a.nhit = e->m.nhit;
+  a.step = e->m.step;
+  a.stream = u1r::STREAM_UPDATE;
+  a.sweep = 0;
+  a.sweep_dev = e->sweep_dev;
+  a.accepts = nullptr;
+  for (int p = 0; p < e->metrop; ++p)
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.pass = (uint32_t)p;
+        a.parity = parity;
+        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_METROPOLIS, mu, e->lat, a, e->upd_blocks, kEnsBlock, G.st));
+      }
+  for (int o = 0; o < e->ovrn; ++o)
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.pass = (uint32_t)o;
+        a.parity = parity;
+        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_OVERRELAX, mu, e->lat, a, e->upd_blocks, kEnsBlock, G.st));
+      }
+  HIPCHK(u1d::launch_increment(e->sweep_dev, G.st));
+  return U1_OK;
+}
+
+void ensemble_release(u1_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->upd_dev) (void)hipFree(e->upd_dev);
+  if (e->meas_dev) (void)hipFree(e->meas_dev);
+  if (e->upd_map) (void)hipFree(e->upd_map);
+  if (e->meas_map) (void)hipFree(e->meas_map);
+  if (e->lat) (void)hipFree(e->lat);
+  if (e->sweep_dev) (void)hipFree(e->sweep_dev);
+  if (e->res_dev) (void)hipFree(e->res_dev);
+  if (e->partials) (void)hipFree(e->partials);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- runtime
+// Start (cuda_error_check.cpp:24-45): select the device, create the stream.
+int u1_init(int device) {
+  if (G.init) return U1_OK;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (n == 0) return fail(U1_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(U1_ERR_ARG, "GPU id out of range");
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreate(&G.st));
+  HIPCHK(hipMalloc((void**)&G.counter, sizeof(unsigned long long)));
+  HIPCHK(hipMalloc((void**)&G.dres, 64 * sizeof(double)));
+  G.device = device;
+  G.init = true;
+  return U1_OK;
+}
+
+// Finalize (cuda_error_check.cpp:47-72) without the exit(): sync and release.
+int u1_finalize(void) {
+  if (!G.init) return U1_OK;
+  HIPCHK(hipStreamSynchronize(G.st));
+  if (G.partials) HIPCHK(hipFree(G.partials));
+  if (G.counter) HIPCHK(hipFree(G.counter));
+  if (G.dres) HIPCHK(hipFree(G.dres));
+  HIPCHK(hipStreamDestroy(G.st));
+  G.partials = nullptr;
+  G.partials_cap = 0;
+  G.counter = nullptr;
+  G.dres = nullptr;
+  G.st = nullptr;
+  G.init = false;
+  return U1_OK;
+}
+
+int u1_sync(void) {
+  REQ_INIT();
+  HIPCHK(hipStreamSynchronize(G.st));
+  return U1_OK;
+}
+
+const char* u1_last_error(void) { return g_err.c_str(); }
+
+int u1_device_name(char* buf, size_t len) {
+  REQ_INIT();
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, G.device));
+  std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- memory
+int u1_malloc(void** ptr, size_t bytes) {
+  REQ_INIT();
+  if (!ptr) return fail(U1_ERR_ARG, "null pointer");
+  *ptr = nullptr;
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 4));
+  HIPCHK(u1d::launch_fill_zero(*ptr, (int64_t)((bytes + 3) / 4 * 4), G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  return U1_OK;
+}
+
+int u1_free(void* ptr) {
+  REQ_INIT();
+  if (!ptr) return U1_OK;
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(ptr));
+  return U1_OK;
+}
+
+int u1_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  REQ_INIT();
+  if ((!dst || !src) && bytes) return fail(U1_ERR_ARG, "null pointer");
+  static const hipMemcpyKind kinds[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                                         hipMemcpyDeviceToDevice};
+  if (kind < 0 || kind > 3) return fail(U1_ERR_ARG, "bad copy kind");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kinds[kind], G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  return U1_OK;
+}
+
+int u1_zero(void* ptr, size_t bytes) {
+  REQ_INIT();
+  if (bytes % 4) return fail(U1_ERR_ARG, "u1_zero needs a multiple of 4 bytes");
+  HIPCHK(u1d::launch_fill_zero(ptr, (int64_t)bytes, G.st));
+  return U1_OK;
+}
+
+int u1_convert_precision(const void* src, int src_prec, void* dst, int dst_prec, int64_t n) {
+  REQ_INIT();
+  RET(check_prec(src_prec));
+  RET(check_prec(dst_prec));
+  HIPCHK(u1d::launch_convert_precision(src, src_prec, dst, dst_prec, n, 256, G.st));
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- updates
+int u1_hotstart(const u1_geom* g, int prec, void* lat, uint64_t seed, uint32_t replica) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  const RepDesc r = make_rep(g, seed, replica);
+  HIPCHK(u1d::launch_hotstart(prec, lat, r, 256, G.st));
+  return U1_OK;
+}
+
+int u1_update(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+              uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t c3, int32_t mask_ml, uint64_t* accepts) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  if (!lat) return fail(U1_ERR_ARG, "null lattice");
+  if (metrop < 0 || ovrn < 0 || metrop > 4095) return fail(U1_ERR_ARG, "bad pass counts");
+  if (mask_ml != 0 && mask_ml != 2 && mask_ml != 4) return fail(U1_ERR_ARG, "mask_ml must be 0, 2 or 4");
+  if (accepts) HIPCHK(u1d::launch_fill_zero(G.counter, sizeof(unsigned long long), G.st));
+  RET(run_update(g, prec, lat, metrop, ovrn, m, seed, stream, sweep, c3, mask_ml, accepts ? G.counter : nullptr));
+  if (accepts) {
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, G.counter, sizeof(v), hipMemcpyDeviceToHost, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+    *accepts += v;
+  }
+  return U1_OK;
+}
+
+int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity, int32_t mu, const u1_metro* m,
+             uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t pass, uint32_t c3, int32_t mask_ml) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  if (!lat) return fail(U1_ERR_ARG, "null lattice");
+  if (kind != 0 && kind != 1) return fail(U1_ERR_ARG, "kind must be 0 (Metropolis) or 1 (overrelaxation)");
+  if (parity < 0 || parity > 1 || mu < 0 || mu > 3) return fail(U1_ERR_ARG, "bad parity or direction");
+  if (mask_ml != 0 && mask_ml != 2 && mask_ml != 4) return fail(U1_ERR_ARG, "mask_ml must be 0, 2 or 4");
+  PhaseArgs a{};
+  a.single = make_rep(g, seed, c3);
+  a.ml = mask_ml;
+  a.proposal = m->proposal;
+  a.nhit = m->nhit;
+  a.step = m->step;
+  a.stream = stream;
+  a.sweep = sweep;
+  a.pass = pass;
+  a.parity = parity;
+  int b = 256;
+  RET(phase_block(prec, kind, g, lat, a, &b));
+  HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, (a.single.HV + b - 1) / b, b, G.st));
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- measurements
+int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(plaquette_async(g, prec, lat, G.dres));
+  return copy_back(out, G.dres, 4);
+}
+
+int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  const RepDesc r = make_rep(g, 0, 0);
+  const int block = 256, nblk = (r.SV + block - 1) / block;
+  RET(ensure_partials(2 * (size_t)nblk));
+  HIPCHK(u1d::launch_polyakov(prec, lat, r, G.partials, nblk, block, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2, 1.0 / (double)r.SV, G.dres, G.st));
+  return copy_back(out, G.dres, 2);
+}
+
+int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
+                   uint32_t stream, uint32_t c1, uint32_t c3, void* P_dev) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad Polyakov mode");
+  if (mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
+  const RepDesc r = make_rep(g, seed, c3);
+  HIPCHK(u1d::launch_poly_volume(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0, mh ? mh->step : 0.0,
+                                 stream, c1, c3, (double*)P_dev, 256, G.st));
+  return U1_OK;
+}
+
+int u1_temporal_links(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh,
+                      uint64_t seed, uint32_t stream, uint32_t c1, uint32_t c3, void* out_dev) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad temporal-link mode");
+  if (mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
+  const RepDesc r = make_rep(g, seed, c3);
+  HIPCHK(u1d::launch_temporal_links(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0,
+                                    mh ? mh->step : 0.0, stream, c1, c3, (double*)out_dev, 256, G.st));
+  return U1_OK;
+}
+
+int u1_pp(const u1_geom* g, const void* P_dev, int32_t rmax, double* out) {
+  REQ_INIT();
+  RET(check_geom(g));
+  if (rmax < 1 || rmax > 32) return fail(U1_ERR_ARG, "rmax must be in [1, 32]");
+  const RepDesc r = make_rep(g, 0, 0);
+  const int block = 256, nblk = (r.SV + block - 1) / block;
+  RET(ensure_partials(2 * (size_t)rmax * nblk));
+  HIPCHK(u1d::launch_pp((const double*)P_dev, r, rmax, G.partials, nblk, block, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres, G.st));
+  return copy_back(out, G.dres, 2 * (size_t)rmax);
+}
+
+int u1_poly2(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
+             uint32_t c1, uint32_t c3, double* out) {
+  REQ_INIT();
+  RET(check_geom(g));
+  const int64_t sv = (int64_t)g->dims[0] * g->dims[1] * g->dims[2];
+  void* P = nullptr;
+  HIPCHK(hipMalloc(&P, 2 * sv * sizeof(double)));
+  int rc = u1_poly_volume(g, prec, lat, mode, mh, seed, U1_STREAM_MULTIHIT, c1, c3, P);
+  if (rc == U1_OK) rc = u1_pp(g, P, g->dims[0] / 2, out);
+  (void)hipStreamSynchronize(G.st);
+  (void)hipFree(P);
+  return rc;
+}
+
+int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32_t k4, int32_t n2, int32_t k2,
+                  int32_t metrop, int32_t ovrn, const u1_metro* m, int32_t mh_mode, const u1_metro* mh,
+                  uint64_t seed, uint32_t call_id, double* out, void* ppspace_dev) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  // multilevel.cu:641-644
+  if (g->dims[3] % 4 != 0) return fail(U1_ERR_ARG, "Cannot apply MultiLevel algorithm: Nt is not a multiple of 4");
+  if (n4 < 1 || n2 < 1 || k4 < 0 || k2 < 0) return fail(U1_ERR_ARG, "bad multilevel level counts");
+  if (mh_mode != U1_POLY_PLAIN && mh_mode != U1_POLY_MULTIHIT && mh_mode != U1_POLY_MULTIHIT_STOCHASTIC)
+    return fail(U1_ERR_ARG, "bad multihit mode");
+  if (mh_mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
+  const RepDesc r = make_rep(g, seed, call_id);
+  const int rmax = r.N[0] / 2;
+  if (rmax > 32) return fail(U1_ERR_ARG, "Nx/2 must be <= 32");
+  const int64_t sz = (int64_t)r.SV * rmax * 3;
+  const int64_t nl2 = r.N[3] / 2, nl4 = r.N[3] / 4;
+  const size_t latb = 4 * (size_t)r.V * elem_size(prec);
+  void *wl = nullptr, *tl = nullptr, *l2 = nullptr, *l4 = nullptr;
+  int rc = U1_OK;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(G.st);
+    if (wl) (void)hipFree(wl);
+    if (tl) (void)hipFree(tl);
+    if (l2) (void)hipFree(l2);
+    if (l4) (void)hipFree(l4);
+  };
+#define MLCHK(expr)                                                                          \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      cleanup();                                                                             \
+      return fail(U1_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    }                                                                                        \
+  } while (0)
+  MLCHK(hipMalloc(&wl, latb));
+  MLCHK(hipMalloc(&tl, 2 * (size_t)r.V * sizeof(double)));
+  MLCHK(hipMalloc(&l2, 2 * (size_t)sz * nl2 * sizeof(double)));
+  MLCHK(hipMalloc(&l4, 2 * (size_t)sz * nl4 * sizeof(double)));
+  MLCHK(hipMemcpyAsync(wl, lat, latb, hipMemcpyDeviceToDevice, G.st));
+  MLCHK(u1d::launch_fill_zero(l4, 2 * sz * nl4 * (int64_t)sizeof(double), G.st));
+  const double l2norm = 1.0 / (double)n2, l4norm = 1.0 / (double)n4;
+  const int mp = mh ? mh->proposal : 0, mn = mh ? mh->nhit : 0;
+  const double ms = mh ? mh->step : 0.0;
+  for (int i = 0; i < n4 && rc == U1_OK; ++i) {
+    for (int j = 0; j < k4 && rc == U1_OK; ++j)
+      rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L4, (uint32_t)(i * k4 + j), call_id, 4,
+                      nullptr);
+    if (rc != U1_OK) break;
+    MLCHK(u1d::launch_fill_zero(l2, 2 * sz * nl2 * (int64_t)sizeof(double), G.st));
+    for (int k = 0; k < n2 && rc == U1_OK; ++k) {
+      for (int l = 0; l < k2 && rc == U1_OK; ++l)
+        rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)((i * n2 + k) * k2 + l),
+                        call_id, 2, nullptr);
+      if (rc != U1_OK) break;
+      MLCHK(u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT,
+                                       (uint32_t)(i * n2 + k), call_id, (double*)tl, 256, G.st));
+      MLCHK(u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st));
+    }
+    if (rc != U1_OK) break;
+    MLCHK(u1d::launch_ml_l4((const double*)l2, (double*)l4, r, rmax, l2norm, 256, G.st));
+  }
+  if (rc == U1_OK) {
+    const int block = 256, nblk = (r.SV + block - 1) / block;
+    rc = ensure_partials(2 * (size_t)rmax * nblk);
+    if (rc == U1_OK) {
+      MLCHK(u1d::launch_ml_final((const double*)l4, (double*)ppspace_dev, r, rmax, l4norm, G.partials, nblk, block,
+                                 G.st));
+      MLCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres, G.st));
+      rc = copy_back(out, G.dres, 2 * (size_t)rmax);
+    }
+  }
+#undef MLCHK
+  cleanup();
+  return rc;
+}
+
+int u1_convert_layout(const u1_geom* g, int prec, const void* in, void* out, int32_t eo_to_no) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  if (in == out) return fail(U1_ERR_ARG, "in and out must differ");
+  const RepDesc r = make_rep(g, 0, 0);
+  HIPCHK(u1d::launch_convert_layout(prec, in, out, r, eo_to_no, 256, G.st));
+  return U1_OK;
+}
+
+int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spacetime, int32_t eo_order,
+                        void* field_dev, double means[12]) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  const RepDesc r = make_rep(g, 0, 0);
+  const int size = spacetime ? r.V : r.SV;
+  const int block = 256, nblk = (size + block - 1) / block;
+  RET(ensure_partials(12 * (size_t)nblk));
+  HIPCHK(u1d::launch_plaquette_fields(prec, lat, r, spacetime, eo_order, (double*)field_dev, G.partials, nblk, block,
+                                      G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 12, 1.0 / (double)size, G.dres, G.st));
+  return copy_back(means, G.dres, 12);
+}
+
+// ---------------------------------------------------------------- ensemble
+int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, const uint64_t* seeds,
+                       const int32_t* hot, int prec, int32_t metrop, int32_t ovrn, const u1_metro* m) {
+  REQ_INIT();
+  if (!out || !geoms || !seeds || nrep < 1 || nrep > 65535) return fail(U1_ERR_ARG, "bad ensemble arguments");
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  *out = nullptr;
+  u1_ensemble* e = new u1_ensemble();
+  e->nrep = nrep;
+  e->prec = prec;
+  e->metrop = metrop;
+  e->ovrn = ovrn;
+  e->m = *m;
+  std::vector<uint16_t> umap, mmap;
+  int64_t off = 0;
+  for (int i = 0; i < nrep; ++i) {
+    int rc = check_geom(&geoms[i]);
+    if (rc != U1_OK) {
+      delete e;
+      return rc;
+    }
+    e->geoms.push_back(geoms[i]);
+    RepDesc r = make_rep(&geoms[i], seeds[i], (uint32_t)i);
+    r.lat_off = (uint64_t)off;
+    off += 4 * (int64_t)r.V;
+    RepDesc u = r, q = r;
+    u.blk_begin = e->upd_blocks;
+    u.nblk = (r.HV + kEnsBlock - 1) / kEnsBlock;
+    e->upd_blocks += u.nblk;
+    for (int b = 0; b < u.nblk; ++b) umap.push_back((uint16_t)i);
+    q.blk_begin = e->meas_blocks;
+    q.nblk = (r.V + kEnsBlock - 1) / kEnsBlock;
+    e->meas_blocks += q.nblk;
+    for (int b = 0; b < q.nblk; ++b) mmap.push_back((uint16_t)i);
+    e->upd_host.push_back(u);
+    e->meas_host.push_back(q);
+  }
+  e->total_links = off;
+  auto bail = [&](const std::string& msg) {
+    ensemble_release(e);
+    delete e;
+    return fail(U1_ERR_HIP, msg);
+  };
+#define ECHK(expr)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return bail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+  ECHK(hipMalloc(&e->lat, (size_t)off * elem_size(prec)));
+  ECHK(u1d::launch_fill_zero(e->lat, off * (int64_t)elem_size(prec), G.st));
+  ECHK(hipMalloc((void**)&e->upd_dev, nrep * sizeof(RepDesc)));
+  ECHK(hipMalloc((void**)&e->meas_dev, nrep * sizeof(RepDesc)));
+  ECHK(hipMalloc((void**)&e->upd_map, umap.size() * sizeof(uint16_t)));
+  ECHK(hipMalloc((void**)&e->meas_map, mmap.size() * sizeof(uint16_t)));
+  ECHK(hipMalloc((void**)&e->sweep_dev, sizeof(uint32_t)));
+  ECHK(hipMalloc((void**)&e->res_dev, 4 * (size_t)nrep * sizeof(double)));
+  ECHK(hipMalloc((void**)&e->partials, 4 * (size_t)e->meas_blocks * sizeof(double)));
+  ECHK(hipMemcpyAsync(e->upd_dev, e->upd_host.data(), nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
+  ECHK(hipMemcpyAsync(e->meas_dev, e->meas_host.data(), nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
+  ECHK(hipMemcpyAsync(e->upd_map, umap.data(), umap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
+  ECHK(hipMemcpyAsync(e->meas_map, mmap.data(), mmap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
+  ECHK(u1d::launch_fill_zero(e->sweep_dev, sizeof(uint32_t), G.st));
+  for (int i = 0; i < nrep; ++i)
+    if (hot && hot[i]) ECHK(u1d::launch_hotstart(prec, e->lat, e->upd_host[i], 256, G.st));
+  ECHK(hipStreamSynchronize(G.st));
+#undef ECHK
+  *out = e;
+  return U1_OK;
+}
+
+int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
+  REQ_INIT();
+  if (!e) return fail(U1_ERR_ARG, "null ensemble");
+  if (use_graph && !e->exec) {
+    HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
+    int rc = ensemble_enqueue_sweep(e);
+    hipGraph_t gr = nullptr;
+    hipError_t ce = hipStreamEndCapture(G.st, &gr);
+    if (rc != U1_OK) return rc;
+    HIPCHK(ce);
+    e->graph = gr;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s < nsweeps; ++s) {
+    if (use_graph)
+      HIPCHK(hipGraphLaunch(e->exec, G.st));
+    else
+      RET(ensemble_enqueue_sweep(e));
+  }
+  e->sweeps_done += (uint64_t)nsweeps;
+  return U1_OK;
+}
+
+int u1_ensemble_plaquette(u1_ensemble* e, double* out) {
+  REQ_INIT();
+  if (!e || !out) return fail(U1_ERR_ARG, "null argument");
+  HIPCHK(u1d::launch_plaquette_batched(e->prec, e->lat, e->meas_dev, e->meas_map, e->partials, e->meas_blocks,
+                                       kEnsBlock, G.st));
+  HIPCHK(u1d::launch_batched_finish(e->partials, e->meas_dev, e->nrep, e->res_dev, G.st));
+  return copy_back(out, e->res_dev, 4 * (size_t)e->nrep);
+}
+
+int u1_ensemble_read(u1_ensemble* e, int32_t r, double* host) {
+  REQ_INIT();
+  if (!e || !host || r < 0 || r >= e->nrep) return fail(U1_ERR_ARG, "bad replica");
+  const RepDesc& d = e->upd_host[r];
+  const int64_t n = 4 * (int64_t)d.V;
+  if (e->prec == U1_F64) {
+    HIPCHK(hipMemcpyAsync(host, (double*)e->lat + d.lat_off, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), (float*)e->lat + d.lat_off, n * sizeof(float), hipMemcpyDeviceToHost, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+    for (int64_t i = 0; i < n; ++i) host[i] = (double)tmp[i];
+  }
+  HIPCHK(hipStreamSynchronize(G.st));
+  return U1_OK;
+}
+
+int u1_ensemble_write(u1_ensemble* e, int32_t r, const double* host) {
+  REQ_INIT();
+  if (!e || !host || r < 0 || r >= e->nrep) return fail(U1_ERR_ARG, "bad replica");
+  const RepDesc& d = e->upd_host[r];
+  const int64_t n = 4 * (int64_t)d.V;
+  if (e->prec == U1_F64) {
+    HIPCHK(hipMemcpyAsync((double*)e->lat + d.lat_off, host, n * sizeof(double), hipMemcpyHostToDevice, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+  } else {
+    std::vector<float> tmp(n);
+    for (int64_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+    HIPCHK(hipMemcpyAsync((float*)e->lat + d.lat_off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+  }
+  return U1_OK;
+}
+
+int u1_ensemble_info(u1_ensemble* e, uint64_t* sweeps_done, int64_t* total_links) {
+  if (!e) return fail(U1_ERR_ARG, "null ensemble");
+  if (sweeps_done) *sweeps_done = e->sweeps_done;
+  if (total_links) *total_links = e->total_links;
+  return U1_OK;
+}
+
+int u1_ensemble_destroy(u1_ensemble* e) {
+  if (!e) return U1_OK;
+  if (G.init) (void)hipStreamSynchronize(G.st);
+  ensemble_release(e);
+  delete e;
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+int u1_tune_enable(int32_t on) {
+  G.tune = on != 0;
+  return U1_OK;
+}
+
+// tunecache.csv format of the reference (tune.cpp:109-122, 238-295):
+//   tunecache <version> <build hash>
+//   volume name aux block.x block.y block.z grid.x grid.y grid.z shared_bytes comment
+static const char* kTuneVersion = "latticeu1_amd-1";
+static const char* kTuneHash = "gpu_arch=gfx950";
+
+int u1_tune_load(const char* path) {
+  if (!path) return fail(U1_ERR_ARG, "null path");
+  std::ifstream f(path);
+  if (!f) return fail(U1_ERR_IO, std::string("cannot open ") + path);
+  std::string line, tok, ver, hash;
+  if (!std::getline(f, line)) return fail(U1_ERR_IO, "empty tune cache");
+  {
+    std::istringstream ls(line);
+    ls >> tok >> ver >> hash;
+    if (tok != "tunecache") return fail(U1_ERR_IO, std::string("Bad format in ") + path);
+    if (ver != kTuneVersion || hash != kTuneHash)
+      return fail(U1_ERR_IO, std::string("Cache file ") + path + " does not match this build");
+  }
+  std::getline(f, line);  // column description
+  int n = 0;
+  while (std::getline(f, line)) {
+    if (line.empty()) continue;
+    std::istringstream ls(line);
+    std::string vol, name, aux;
+    int bx, by, bz, gx, gy, gz, sh;
+    if (!(std::getline(ls, vol, '\t') && std::getline(ls, name, '\t') && std::getline(ls, aux, '\t')))
+      return fail(U1_ERR_IO, std::string("Bad format in ") + path);
+    if (!(ls >> bx >> by >> bz >> gx >> gy >> gz >> sh)) return fail(U1_ERR_IO, std::string("Bad format in ") + path);
+    G.tunecache[vol + "\t" + name] = bx;
+    ++n;
+  }
+  G.tune_loaded = n;
+  return U1_OK;
+}
+
+int u1_tune_save(const char* path) {
+  if (!path) return fail(U1_ERR_ARG, "null path");
+  std::ofstream f(path);
+  if (!f) return fail(U1_ERR_IO, std::string("cannot write ") + path);
+  f << "tunecache\t" << kTuneVersion << "\t" << kTuneHash << "\n";
+  f << "volume\tname\taux\tblock.x\tblock.y\tblock.z\tgrid.x\tgrid.y\tgrid.z\tshared_bytes\tcomment\n";
+  for (const auto& kv : G.tunecache) {
+    const std::string& key = kv.first;
+    const size_t tab = key.find('\t');
+    const std::string vol = key.substr(0, tab), name = key.substr(tab + 1);
+    f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second << "\t1\t1\t0\t1\t1\t0\t# workgroup size\n";
+  }
+  return f ? U1_OK : fail(U1_ERR_IO, "write failed");
+}
+
+int u1_tune_entries(int32_t* n) {
+  if (!n) return fail(U1_ERR_ARG, "null pointer");
+  *n = (int32_t)G.tunecache.size();
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- timing helper
+int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+                     uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
+                     double* elapsed_ms, double* kernel_ms) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  if (nsweeps < 1) return fail(U1_ERR_ARG, "nsweeps must be >= 1");
+  double* series = nullptr;
+  if (measure_plaq) HIPCHK(hipMalloc((void**)&series, 4 * (size_t)nsweeps * sizeof(double)));
+  {  // tune outside the timed region
+    PhaseArgs a{};
+    a.single = make_rep(g, seed, 0);
+    a.nhit = m->nhit;
+    a.proposal = m->proposal;
+    a.step = m->step;
+    int b;
+    if (metrop > 0) RET(phase_block(prec, u1d::KIND_METROPOLIS, g, lat, a, &b));
+    if (ovrn > 0) RET(phase_block(prec, u1d::KIND_OVERRELAX, g, lat, a, &b));
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipEventRecord(e0, G.st));
+  for (int s = 0; s < nsweeps; ++s) {
+    RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)s, 0, 0, nullptr));
+    if (measure_plaq) RET(plaquette_async(g, prec, lat, series + 4 * s));
+  }
+  HIPCHK(hipEventRecord(e1, G.st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *elapsed_ms = ms;
+  if (measure_plaq && plaq_series)
+    HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));
+  if (series) HIPCHK(hipFree(series));
+  if (kernel_ms) {
+    // per-kernel durations: start/stop events around every phase launch
+    const int nk = 8 * (metrop + ovrn);
+    const int ns = std::min(nsweeps, 16);
+    std::vector<hipEvent_t> ev(2 * (size_t)nk);
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    double sum[2] = {0, 0};
+    long cnt[2] = {0, 0};
+    for (int s = 0; s < ns; ++s) {
+      std::vector<std::pair<int, int>> kinds;
+      RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)(nsweeps + s), 0, 0,
+                     nullptr, ev.data(), &kinds));
+      HIPCHK(hipEventSynchronize(ev.back()));
+      for (auto& kd : kinds) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, ev[2 * kd.first], ev[2 * kd.first + 1]));
+        sum[kd.second] += t;
+        cnt[kd.second] += 1;
+      }
+    }
+    for (auto& x : ev) HIPCHK(hipEventDestroy(x));
+    kernel_ms[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;
+    kernel_ms[1] = cnt[1] ? sum[1] / cnt[1] : 0.0;
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  return U1_OK;
+}
+
+// ---------------------------------------------------------------- self test
+int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n) {
+  REQ_INIT();
+  if (fn < 0 || fn > 6 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
+  if (n == 0) return U1_OK;
+  const int nout = (fn == 0 || fn == 3 || fn == 5) ? 2 : 1;
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc((void**)&dx, n * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dy, n * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dout, nout * n * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(dy, y ? y : x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
+  HIPCHK(u1d::launch_selftest_math(fn, dx, dy, dout, n, G.st));
+  HIPCHK(hipMemcpyAsync(out, dout, nout * n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(dx));
+  HIPCHK(hipFree(dy));
+  HIPCHK(hipFree(dout));
+  return U1_OK;
+}
+
+int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n) {
+  REQ_INIT();
+  if (!ctr || !key || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
+  if (n == 0) return U1_OK;
+  uint32_t *dc = nullptr, *dk = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc((void**)&dc, 4 * n * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&dk, 2 * n * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&dout, 4 * n * sizeof(uint32_t)));
+  HIPCHK(hipMemcpyAsync(dc, ctr, 4 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(dk, key, 2 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(u1d::launch_selftest_philox(dc, dk, dout, n, G.st));
+  HIPCHK(hipMemcpyAsync(out, dout, 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(dc));
+  HIPCHK(hipFree(dk));
+  HIPCHK(hipFree(dout));
+  return U1_OK;
+}
+
+}  // extern "C"

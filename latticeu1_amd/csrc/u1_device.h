@@ -1,0 +1,150 @@
+// u1_device.h -- device-side building blocks of the gfx950 path.
+//
+// Lattice layout in HBM (same even/odd structure-of-arrays order as the
+// reference, index.h:124-132, array.h): one contiguous plane per direction mu,
+// each plane split into the even and the odd half,
+//     angle(x, mu) = lat[mu*V + parity(x)*V/2 + (lex(x) >> 1)],
+//     lex(x) = x0 + Nx*(x1 + Ny*(x2 + Nz*x3)).
+// A wavefront of 64 consecutive eo-ids walks consecutive addresses of every
+// plane it touches, so each of the 19 angle loads of a link update is a fully
+// coalesced 256 B (FP32) or 512 B (FP64) request.  Neighbour eo-ids are
+// separable: eo(x + a e_d + b e_e) = eo(x) + off_d(a) + off_e(b) for d != e,
+// so eight per-site offsets (forward/backward per direction) give all twelve
+// neighbours of a staple with adds only (no division per neighbour).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "u1_math.h"
+#include "u1_philox.h"
+
+namespace u1d {
+
+// Per-lattice (per-replica) descriptor, read by every workgroup through the
+// scalar cache.  Batched launches index a table of these.
+struct RepDesc {
+  int32_t N[4];        // Nx, Ny, Nz, Nt
+  int32_t nxh;         // Nx / 2
+  int32_t HV, V;       // half volume, volume
+  int32_t SV;          // spatial volume
+  int32_t sy, sz, st;  // eo strides of +y, +z, +t: Nx/2, Nx*Ny/2, Nx*Ny*Nz/2
+  int32_t blk_begin;   // first workgroup of this replica in a batched launch
+  int32_t nblk;        // workgroups of this replica
+  uint32_t c3;         // Philox counter word 3 (replica id)
+  uint32_t div_nxh_m, div_ny_m, div_nz_m;  // magic multipliers for id/nxh, /Ny, /Nz
+  uint32_t div_nxh_s, div_ny_s, div_nz_s;  // and their shifts
+  uint32_t pad0;
+  double beta;
+  double aniso;
+  uint64_t seed;
+  uint64_t lat_off;  // element offset of this replica's angles in the packed buffer
+};
+
+// Round-up magic division (Granlund & Montgomery 1994): for 1 <= d < 2^31 and
+// n < 2^31, n / d = (umulhi(n, m) + n) >> s with s = ceil(log2 d),
+// m = floor(2^32 (2^s - d) / d) + 1.
+struct MagicDiv {
+  uint32_t m, s;
+};
+inline MagicDiv make_magic(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  const uint64_t m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+  return {(uint32_t)m, s};
+}
+__host__ __device__ __forceinline__ uint32_t mdiv(uint32_t n, uint32_t m, uint32_t s) {
+  return ((uint32_t)(((uint64_t)n * m) >> 32) + n) >> s;  // v_mul_hi_u32 on the device
+}
+
+// Site coordinates and the eight neighbour offsets of the even/odd site
+// (id, parity) -- the strength-reduced form of indexEO / indexEO_neg
+// (index.h:114-153).
+struct Site {
+  int32_t x[4];
+  int32_t fwd[4], bwd[4];
+};
+
+__host__ __device__ __forceinline__ void site_of(const RepDesc& r, int32_t id, int32_t parity, Site& s) {
+  const uint32_t row = mdiv((uint32_t)id, r.div_nxh_m, r.div_nxh_s);
+  const int32_t xh = id - (int32_t)row * r.nxh;
+  const uint32_t zt = mdiv(row, r.div_ny_m, r.div_ny_s);
+  const int32_t y = (int32_t)row - (int32_t)zt * r.N[1];
+  const uint32_t t = mdiv(zt, r.div_nz_m, r.div_nz_s);
+  const int32_t z = (int32_t)zt - (int32_t)t * r.N[2];
+  const int32_t xodd = (y + z + (int32_t)t + parity) & 1;
+  s.x[0] = 2 * xh + xodd;
+  s.x[1] = y;
+  s.x[2] = z;
+  s.x[3] = (int32_t)t;
+  // x direction: the eo-id moves only when crossing a pair boundary
+  s.fwd[0] = xodd ? ((xh == r.nxh - 1) ? -(r.nxh - 1) : 1) : 0;
+  s.bwd[0] = xodd ? 0 : ((xh == 0) ? (r.nxh - 1) : -1);
+  s.fwd[1] = (y == r.N[1] - 1) ? -(r.N[1] - 1) * r.sy : r.sy;
+  s.bwd[1] = (y == 0) ? (r.N[1] - 1) * r.sy : -r.sy;
+  s.fwd[2] = (z == r.N[2] - 1) ? -(r.N[2] - 1) * r.sz : r.sz;
+  s.bwd[2] = (z == 0) ? (r.N[2] - 1) * r.sz : -r.sz;
+  s.fwd[3] = ((int32_t)t == r.N[3] - 1) ? -(r.N[3] - 1) * r.st : r.st;
+  s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
+}
+
+// ------------------------------------------------------- precision traits
+template <typename Real>
+struct Prec;
+
+template <>
+struct Prec<double> {
+  static __device__ __forceinline__ void sincos(double x, double* s, double* c) { u1m::sincos_d(x, s, c); }
+  static __device__ __forceinline__ double exp(double x) { return u1m::exp_d(x); }
+  static __device__ __forceinline__ double atan2(double y, double x) { return u1m::atan2_d(y, x); }
+  static __device__ __forceinline__ double u01(uint32_t w) { return u1r::u01_d(w); }
+};
+
+template <>
+struct Prec<float> {
+  static __device__ __forceinline__ void sincos(float x, float* s, float* c) { u1m::sincos_f(x, s, c); }
+  // Hardware exp2 (v_exp_f32); used only for the Metropolis weight, whose
+  // FP32 parity bar is statistical.
+  static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+  static __device__ __forceinline__ float atan2(float y, float x) { return u1m::atan2_f(y, x); }
+  static __device__ __forceinline__ float u01(uint32_t w) { return u1r::u01_f(w); }
+};
+
+// ------------------------------------------------------- wave64 reductions
+// Butterfly over the 64 lanes of a wavefront (ds_bpermute based shuffles);
+// fixed order, so results are reproducible run to run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum of NV doubles per thread into out[0..NV) (valid in thread 0).
+// Partial sums of the waves go through LDS and are added in wave order.
+template <int NV, int MAXW = 16>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
+  __shared__ double red[MAXW][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < nw; ++i) s += red[i][k];
+      out[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace u1d

@@ -1,0 +1,79 @@
+// u1_kernels.h -- host-side launchers of the gfx950 kernels (internal to the
+// library; the public boundary is include/latticeu1_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "u1_device.h"
+
+namespace u1d {
+
+// One checkerboard (parity, mu) phase of Metropolis or overrelaxation over one
+// lattice, or over every replica of a batch in a single launch.
+struct PhaseArgs {
+  RepDesc single;               // used when reps == nullptr
+  const RepDesc* reps;          // batched: replica table (device)
+  const uint16_t* blockmap;     // batched: workgroup -> replica (device)
+  int32_t parity;
+  int32_t ml;                   // multilevel mask: 0 none, 2 or 4
+  int32_t proposal;             // 0 independent, 1 theta + delta
+  int32_t nhit;
+  double step;
+  uint32_t stream, pass, sweep;
+  uint32_t pad;
+  const uint32_t* sweep_dev;    // if set, the sweep index is read here (graph replays)
+  unsigned long long* accepts;  // if set, accepted hits are added here
+};
+
+enum { KIND_METROPOLIS = 0, KIND_OVERRELAX = 1 };
+enum { PREC_F64 = 0, PREC_F32 = 1 };
+
+hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
+                        hipStream_t st);
+
+hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t st);
+
+// measurement launchers (u1_measure.hip)
+// partial sums per workgroup; finish_sums folds them in workgroup order.
+hipError_t launch_plaquette(int prec, const void* lat, const RepDesc& r, double* partials /*4*nblk*/, int nblk,
+                            int block, hipStream_t st);
+hipError_t launch_polyakov(int prec, const void* lat, const RepDesc& r, double* partials /*2*nblk*/, int nblk,
+                           int block, hipStream_t st);
+hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double scale, double* out, hipStream_t st);
+hipError_t launch_poly_volume(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
+                              double step, uint32_t stream, uint32_t c1, uint32_t c3, double* P, int block,
+                              hipStream_t st);
+hipError_t launch_temporal_links(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
+                                 double step, uint32_t stream, uint32_t c1, uint32_t c3, double* out, int block,
+                                 hipStream_t st);
+hipError_t launch_pp(const double* P, const RepDesc& r, int rmax, double* partials /*2*rmax*nblk*/, int nblk,
+                     int block, hipStream_t st);
+hipError_t launch_convert_layout(int prec, const void* in, void* out, const RepDesc& r, int eo_to_no, int block,
+                                 hipStream_t st);
+hipError_t launch_convert_precision(const void* in, int in_prec, void* out, int out_prec, int64_t n, int block,
+                                    hipStream_t st);
+hipError_t launch_plaquette_fields(int prec, const void* lat, const RepDesc& r, int spacetime, int eo_order,
+                                   double* field, double* partials /*12*nblk*/, int nblk, int block,
+                                   hipStream_t st);
+hipError_t launch_fill_zero(void* p, int64_t bytes, hipStream_t st);
+
+// multilevel (u1_multilevel.hip)
+hipError_t launch_ml_l2(const double* tl, double* l2, const RepDesc& r, int rmax, int block, hipStream_t st);
+hipError_t launch_ml_l4(const double* l2, double* l4, const RepDesc& r, int rmax, double l2norm, int block,
+                        hipStream_t st);
+hipError_t launch_ml_final(const double* l4, double* ppspace, const RepDesc& r, int rmax, double l4norm,
+                           double* partials /*2*rmax*nblk*/, int nblk, int block, hipStream_t st);
+
+// batched replica plaquettes: one partial row per workgroup, tagged by replica.
+hipError_t launch_plaquette_batched(int prec, const void* lat_base, const RepDesc* reps, const uint16_t* blockmap,
+                                    double* partials /*4*nblk*/, int nblk, int block, hipStream_t st);
+hipError_t launch_batched_finish(const double* partials, const RepDesc* reps, int nrep, double* out /*4*nrep*/,
+                                 hipStream_t st);
+hipError_t launch_increment(uint32_t* counter, hipStream_t st);
+
+// self test (u1_selftest.hip)
+hipError_t launch_selftest_math(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t st);
+hipError_t launch_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n,
+                                  hipStream_t st);
+
+}  // namespace u1d

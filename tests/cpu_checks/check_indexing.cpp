@@ -1,0 +1,70 @@
+// CPU check of the device-side indexing (u1_device.h: make_magic, mdiv,
+// site_of): for every even/odd site of several geometries, the coordinates and
+// the twelve staple neighbour positions the update kernels form
+//   opp + fwd[mu], opp + fwd[nu], same, same + fwd[mu] + bwd[nu], opp + bwd[nu]
+// must equal the brute-force positions from explicit coordinates and lie inside
+// the half-plane [0, V/2).  Also checks the magic division exhaustively for
+// small divisors.  Host-only program (no kernel launch).
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../latticeu1_amd/csrc/u1_device.h"
+
+static int lexi(const int* N, const int* x) { return x[0] + N[0] * (x[1] + N[1] * (x[2] + N[2] * x[3])); }
+
+int main() {
+  // magic division
+  for (uint32_t d = 1; d <= 300; ++d) {
+    const u1d::MagicDiv md = u1d::make_magic(d);
+    for (uint32_t n = 0; n < 200000; n += 7)
+      if (u1d::mdiv(n, md.m, md.s) != n / d) { printf("mdiv fail d=%u n=%u\n", d, n); return 1; }
+    for (uint32_t n = (1u << 31) - 5000; n < (1u << 31); ++n)
+      if (u1d::mdiv(n, md.m, md.s) != n / d) { printf("mdiv fail d=%u n=%u\n", d, n); return 1; }
+  }
+  const int geoms[][4] = {{2, 2, 2, 2}, {4, 4, 4, 4}, {8, 8, 8, 8}, {6, 10, 4, 12}, {4, 4, 4, 8},
+                          {12, 12, 12, 16}, {24, 24, 24, 32}, {16, 16, 16, 32}, {48, 48, 48, 48}};
+  long checked = 0;
+  for (const auto& G : geoms) {
+    u1d::RepDesc r{};
+    for (int i = 0; i < 4; ++i) r.N[i] = G[i];
+    r.nxh = G[0] / 2;
+    r.V = G[0] * G[1] * G[2] * G[3];
+    r.HV = r.V / 2;
+    r.sy = r.nxh; r.sz = r.nxh * G[1]; r.st = r.nxh * G[1] * G[2];
+    u1d::MagicDiv a = u1d::make_magic(r.nxh), b = u1d::make_magic(G[1]), c = u1d::make_magic(G[2]);
+    r.div_nxh_m = a.m; r.div_nxh_s = a.s; r.div_ny_m = b.m; r.div_ny_s = b.s; r.div_nz_m = c.m; r.div_nz_s = c.s;
+    const int step = r.V > 1000000 ? 13 : 1;
+    for (int parity = 0; parity < 2; ++parity)
+      for (int id = 0; id < r.HV; id += step) {
+        u1d::Site s;
+        u1d::site_of(r, id, parity, s);
+        for (int k = 0; k < 4; ++k)
+          if (s.x[k] < 0 || s.x[k] >= G[k]) { printf("coord out of range\n"); return 1; }
+        if ((lexi(G, s.x) >> 1) != id || ((s.x[0] + s.x[1] + s.x[2] + s.x[3]) & 1) != parity) {
+          printf("coords wrong %d %d\n", id, parity); return 1;
+        }
+        for (int mu = 0; mu < 4; ++mu)
+          for (int nu = 0; nu < 4; ++nu) {
+            if (nu == mu) continue;
+            auto pos = [&](int dm, int dn) {
+              int y[4] = {s.x[0], s.x[1], s.x[2], s.x[3]};
+              y[mu] = (y[mu] + dm + G[mu]) % G[mu];
+              y[nu] = (y[nu] + dn + G[nu]) % G[nu];
+              return lexi(G, y) >> 1;
+            };
+            const int got[5] = {id + s.fwd[mu], id + s.fwd[nu], id, id + s.fwd[mu] + s.bwd[nu], id + s.bwd[nu]};
+            const int want[5] = {pos(1, 0), pos(0, 1), id, pos(1, -1), pos(0, -1)};
+            for (int k = 0; k < 5; ++k) {
+              if (got[k] != want[k] || got[k] < 0 || got[k] >= r.HV) {
+                printf("neighbour %d wrong: G=%dx%dx%dx%d id=%d parity=%d mu=%d nu=%d got=%d want=%d\n", k, G[0], G[1],
+                       G[2], G[3], id, parity, mu, nu, got[k], want[k]);
+                return 1;
+              }
+            }
+            ++checked;
+          }
+      }
+  }
+  printf("indexing ok (%ld site/direction pairs)\n", checked);
+  return 0;
+}

@@ -1,0 +1,74 @@
+"""Host-side checks of the C ABI library (no GPU needed, no compute calls).
+
+* libu1amd.so loads and exports every function include/latticeu1_amd.h declares;
+* calls before u1_init fail with U1_ERR_STATE and an error message;
+* the tune cache file round-trips and malformed files are rejected
+  (tune.cpp:195-209 treats a bad cache as an error).
+"""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "latticeu1_amd.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(u1_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for must in ["u1_init", "u1_update", "u1_hotstart", "u1_plaquette", "u1_polyakov", "u1_poly2", "u1_multilevel",
+                 "u1_convert_layout", "u1_plaquette_fields", "u1_ensemble_create", "u1_tune_save"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import latticeu1_amd as u1
+
+    lib = u1.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python mirror binds every declared entry point
+    bound = set(u1._SIGS) | {"u1_last_error"}
+    assert set(declared_functions()) <= bound
+
+
+def test_calls_before_init_fail_cleanly():
+    import latticeu1_amd as u1
+
+    lib = u1.load()
+    g = u1.geom((4, 4, 4, 4))
+    out = (C.c_double * 4)()
+    assert lib.u1_plaquette(C.byref(g), 0, None, out) == 3
+    assert b"u1_init" in lib.u1_last_error()
+    with pytest.raises(u1.U1Error):
+        u1.call("u1_update", C.byref(g), 0, None, 1, 3, C.byref(u1.BASELINE_METRO), 1, 0, 0, 0, 0, None)
+
+
+def test_tune_cache_roundtrip(tmp_path):
+    import latticeu1_amd as u1
+
+    lib = u1.load()
+    p = tmp_path / "tunecache.csv"
+    p.write_text("tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n"
+                 "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t0\t1\t1\t0\t# x\n")
+    u1.tune_load(str(p))
+    n = C.c_int32(0)
+    assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 1
+    q = tmp_path / "out.csv"
+    u1.tune_save(str(q))
+    txt = q.read_text()
+    assert txt.startswith("tunecache\tlatticeu1_amd-1") and "8x8x8x8\tMetropolis_f64" in txt
+    bad = tmp_path / "bad.csv"
+    bad.write_text("not a cache\n")
+    with pytest.raises(u1.U1Error):
+        u1.tune_load(str(bad))
+    stale = tmp_path / "stale.csv"
+    stale.write_text("tunecache\tother-version\tgpu_arch=gfx950\n")
+    with pytest.raises(u1.U1Error):
+        u1.tune_load(str(stale))

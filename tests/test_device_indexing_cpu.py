@@ -1,0 +1,26 @@
+"""CPU check of the device-side index arithmetic before it runs on the GPU.
+
+Builds tests/cpu_checks/check_indexing.cpp (host-only, includes the kernels'
+u1_device.h) and runs it: magic division against '/', and every staple
+neighbour position of every site of several geometries (2^4 ... 48^4) against
+brute-force coordinates and the half-plane bounds.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_device_indexing_on_cpu(tmp_path):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = tmp_path / "check_indexing"
+    subprocess.run([hipcc, "-O2", "-std=c++17", os.path.join(HERE, "cpu_checks", "check_indexing.cpp"), "-o",
+                    str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "indexing ok" in out.stdout

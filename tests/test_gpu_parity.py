@@ -1,0 +1,319 @@
+"""GPU parity of the gfx950 path against the FP64 host oracle, through the C ABI.
+
+Bars (written in each test):
+  FP64 updates      bit-identical lattices and identical accept counts after
+                    every sweep; plaquette within 1e-12 (only the summation
+                    order of the FP64 reductions differs)
+  FP64 measurements relative 1e-12 (Polyakov, correlators, multihit,
+                    multilevel: 1e-11)
+  FP32 bulk         fixed configurations: relative 1e-6 (plaquette, correlators)
+  layout / fields   exact / 1e-13
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def make_pair(u1, orc, dims, beta=1.0, aniso=1.0, hot=True, seed=1, prec=0):
+    g = u1.geom(dims, beta, aniso)
+    og = orc.geom(dims, beta, aniso)
+    lat = u1.Lattice.cold(g, prec)
+    if hot:
+        lat.hotstart(seed)
+        ref = orc.hotstart(og, seed)
+    else:
+        ref = orc.cold(og)
+    return g, og, lat, ref
+
+
+# ------------------------------------------------------------ shared arithmetic
+def test_device_math_bit_identical(u1, orc):
+    say("device math self test")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-60, 60, 20000), rng.uniform(-1e-6, 1e-6, 100), [0.0, -0.0, 1e-300, np.pi]])
+    y = rng.uniform(-5, 5, x.size)
+    d = u1.selftest_math(0, x)
+    s, c = orc.sincos(x)
+    assert np.array_equal(bits(d[:, 0]), bits(s)) and np.array_equal(bits(d[:, 1]), bits(c))
+    xe = x / 2
+    assert np.array_equal(bits(u1.selftest_math(1, xe)), bits(orc.exp(xe)))
+    assert np.array_equal(bits(u1.selftest_math(2, x, y)), bits(orc.atan2(x, y)))
+    xb = np.abs(x[:2000])
+    b = u1.selftest_math(5, xb)
+    i0, i1 = orc.bessel_i0i1(xb)
+    assert np.array_equal(bits(b[:, 0]), bits(i0)) and np.array_equal(bits(b[:, 1]), bits(i1))
+    w = rng.uniform(-3 * np.pi, 3 * np.pi, 5000)
+    wd = u1.selftest_math(6, w)
+    assert wd.min() >= -np.pi and wd.max() < np.pi
+    # FP32 sincos / atan2 accuracy (these are not bit-matched to the host)
+    xf = rng.uniform(-20, 20, 20000).astype(np.float32).astype(np.float64)
+    f = u1.selftest_math(3, xf)
+    assert np.max(np.abs(f[:, 0] - np.sin(xf))) < 2.5e-7 and np.max(np.abs(f[:, 1] - np.cos(xf))) < 2.5e-7
+    yf = rng.uniform(-5, 5, 20000).astype(np.float32).astype(np.float64)
+    assert np.max(np.abs(u1.selftest_math(4, xf, yf) - np.arctan2(xf, yf))) < 5e-7
+
+
+def test_device_philox_kat(u1):
+    from tests.test_oracle_math import KAT
+
+    ctr = np.array([c for c, _, _ in KAT], dtype=np.uint32)
+    key = np.array([k for _, k, _ in KAT], dtype=np.uint32)
+    out = u1.selftest_philox(ctr, key)
+    for i, (_, _, exp) in enumerate(KAT):
+        assert tuple(int(v) for v in out[i]) == exp
+
+
+# ------------------------------------------------------------ FP64 update parity
+@pytest.mark.parametrize("start", ["cold", "hot"])
+@pytest.mark.parametrize("mode", ["baseline", "reference"])
+def test_fp64_sweep_parity_8x4(u1, orc, start, mode):
+    """8^4, beta = 1, 200 sweeps of (1 Metropolis + 3 OR); baseline proposal
+    theta + U[-0.5,0.5) with 4 hits, or the reference's independent proposal."""
+    dims = (8, 8, 8, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, start == "hot", seed=1234)
+    m = u1.BASELINE_METRO if mode == "baseline" else u1.REFERENCE_METRO
+    om = orc.metro(m.proposal, m.nhit, m.step)
+    say(f"fp64 parity 8^4 start={start} mode={mode}")
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    worst = 0.0
+    for s in range(200):
+        a_gpu = lat.update(1, 3, m, 1234, s, count_accepts=True)
+        a_orc = orc.update(og, ref, 1, 3, om, 1234, s, count_accepts=True)
+        assert a_gpu == a_orc, f"sweep {s}: accepts {a_gpu} != {a_orc}"
+        host = lat.to_host()
+        if not np.array_equal(bits(host), bits(ref)):
+            bad = np.nonzero(bits(host) != bits(ref))[0]
+            pytest.fail(f"sweep {s}: {bad.size} links differ, first {bad[:5]}")
+        p_gpu, p_orc = lat.plaquette(), orc.plaquette(og, ref)
+        worst = max(worst, float(np.max(np.abs(p_gpu - p_orc))))
+        assert worst <= 1e-12, f"sweep {s}: plaquette {p_gpu} vs {p_orc}"
+        if s % 50 == 49:
+            say(f"  sweep {s + 1}: <P> = {u1.mean_plaquette(p_gpu):.12f}, max |dplaq| = {worst:.2e}")
+
+
+@pytest.mark.parametrize("dims,beta,aniso,metrop,ovrn,nsw", [
+    ((4, 4, 4, 8), 1.5, 2.0, 1, 3, 30),      # anisotropic (space-time staples weighted xi)
+    ((6, 10, 4, 12), 0.9, 1.0, 2, 1, 20),    # extents that are not powers of two, 2 Metropolis passes
+    ((2, 2, 2, 2), 1.0, 1.0, 1, 3, 20),      # smallest lattice: every neighbour wraps
+])
+def test_fp64_parity_geometries(u1, orc, dims, beta, aniso, metrop, ovrn, nsw):
+    g, og, lat, ref = make_pair(u1, orc, dims, beta, aniso, True, seed=77)
+    say(f"fp64 parity {dims} beta={beta} xi={aniso}")
+    m = u1.BASELINE_METRO
+    om = orc.metro(1, 4, 0.5)
+    for s in range(nsw):
+        assert lat.update(metrop, ovrn, m, 77, s, count_accepts=True) == \
+            orc.update(og, ref, metrop, ovrn, om, 77, s, count_accepts=True)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    assert np.max(np.abs(lat.plaquette() - orc.plaquette(og, ref))) <= 1e-12
+
+
+def test_phase_write_set_gpu(u1, orc):
+    """Each (parity, mu) launch modifies only its own half-plane."""
+    dims = (8, 4, 6, 4)
+    g, og, lat, ref = make_pair(u1, orc, dims, seed=5)
+    V = int(np.prod(dims))
+    for kind in (0, 1):
+        for parity in (0, 1):
+            for mu in range(4):
+                before = lat.to_host()
+                lat.phase(kind, parity, mu, u1.BASELINE_METRO, 5, sweep=kind)
+                after = lat.to_host()
+                ch = np.nonzero(bits(after) != bits(before))[0]
+                lo = mu * V + parity * (V // 2)
+                assert ch.size > 0 and ch.min() >= lo and ch.max() < lo + V // 2
+
+
+def test_masked_multilevel_update_parity(u1, orc):
+    dims = (4, 4, 4, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, seed=9)
+    for ml, stream in ((4, 3), (2, 4)):
+        for s in range(3):
+            lat.update(1, 3, u1.BASELINE_METRO, 9, s, stream=stream, c3=5, mask_ml=ml)
+            orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 9, s, stream=stream, c3=5, mask_ml=ml)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+
+
+# ------------------------------------------------------------ FP64 measurements
+@pytest.fixture(scope="module")
+def thermal(u1, orc):
+    """A thermalised 8^3 x 8 FP64 configuration (GPU path, bit-equal to the oracle's)."""
+    dims = (8, 8, 8, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=4321)
+    for s in range(40):
+        lat.update(1, 3, u1.BASELINE_METRO, 4321, s)
+    for s in range(40):
+        orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 4321, s)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    return g, og, lat, ref
+
+
+def test_polyakov_and_correlators_fp64(u1, orc, thermal):
+    g, og, lat, ref = thermal
+    say("fp64 measurements")
+    L_gpu, L_orc = lat.polyakov(), orc.polyakov(og, ref)
+    assert abs(L_gpu - L_orc) <= 1e-12 * max(1.0, abs(L_orc))
+    for mode in (0, 1):
+        P_gpu, P_orc = lat.poly_volume(mode), orc.poly_volume(og, ref, mode)
+        assert np.allclose(P_gpu, P_orc, rtol=1e-12, atol=1e-15)
+        c_gpu, c_orc = lat.poly2(mode), orc.poly2(og, ref, mode)
+        assert np.allclose(c_gpu, c_orc, rtol=1e-12, atol=1e-15)
+    mh = u1.metro(1, 100, 0.5)
+    c_gpu = lat.poly2(2, mh, seed=11, c1=3, c3=1)
+    c_orc = orc.poly2(og, ref, 2, orc.metro(1, 100, 0.5), seed=11, c1=3, c3=1)
+    assert np.allclose(c_gpu, c_orc, rtol=1e-11, atol=1e-15)
+    for mode in (0, 1, 2):
+        t_gpu = lat.temporal_links(mode, mh, seed=3, c1=2)
+        t_orc = orc.temporal_links(og, ref, mode, orc.metro(1, 100, 0.5), seed=3, c1=2)
+        assert np.allclose(t_gpu, t_orc, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("mh_mode", [1, 2])
+def test_multilevel_fp64(u1, orc, mh_mode):
+    dims = (4, 4, 4, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=19)
+    for s in range(5):
+        lat.update(1, 3, u1.BASELINE_METRO, 19, s)
+        orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 19, s)
+    say(f"multilevel fp64 mh_mode={mh_mode}")
+    mh = u1.metro(1, 20, 0.5)
+    r_gpu, f_gpu = lat.multilevel(2, 1, 3, 1, 1, 2, u1.BASELINE_METRO, 19, call_id=2, mh_mode=mh_mode, mh=mh,
+                                  field=True)
+    r_orc, f_orc = orc.multilevel(og, ref, 2, 1, 3, 1, 1, 2, orc.metro(1, 4, 0.5), 19, call_id=2, mh_mode=mh_mode,
+                                  mh=orc.metro(1, 20, 0.5), field=True)
+    assert np.allclose(r_gpu, r_orc, rtol=1e-11, atol=1e-16)
+    assert np.allclose(f_gpu, f_orc, rtol=1e-11, atol=1e-16)
+    # lattice passed in is not modified (multilevel works on a copy)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+
+
+def test_multilevel_errors(u1):
+    g = u1.geom((4, 4, 4, 6))
+    lat = u1.Lattice.cold(g)
+    with pytest.raises(u1.U1Error):
+        lat.multilevel(1, 1, 1, 1, 1, 1, u1.BASELINE_METRO, 1)
+    with pytest.raises(u1.U1Error):
+        u1.Lattice.cold(u1.geom((4, 4, 3, 4)))  # allocation is fine...
+        bad = u1.Lattice(u1.geom((4, 4, 3, 4)), 0, u1.DeviceArray(4 * 48))
+        bad.plaquette()  # ...but odd extents are rejected (parameters.cu:163-168)
+
+
+def test_layout_and_fields_fp64(u1, orc, thermal):
+    g, og, lat, ref = thermal
+    no = lat.convert_layout(True)
+    assert np.array_equal(bits(no.to_host()), bits(orc.convert(og, ref, True)))
+    back = no.convert_layout(False)
+    assert np.array_equal(bits(back.to_host()), bits(ref))
+    for st in (True, False):
+        F, m = lat.plaquette_fields(st, True)
+        Fo, mo = orc.plaquette_fields(og, ref, st, True)
+        assert np.allclose(F, Fo, atol=1e-13) and np.allclose(m, mo, atol=1e-13)
+        Fn, mn = no.plaquette_fields(st, False)
+        assert np.allclose(Fn, Fo, atol=1e-13) and np.allclose(mn, mo, atol=1e-13)
+
+
+# ------------------------------------------------------------ FP32 bulk
+def test_fp32_fixed_configuration(u1, orc, thermal):
+    """FP32 angles (FP64 reductions) on a fixed FP64 configuration: relative 1e-6."""
+    g, og, lat, ref = thermal
+    f32 = lat.astype(u1.F32)
+    say("fp32 fixed configuration")
+    p32, p64 = f32.plaquette(), orc.plaquette(og, ref)
+    for k in (0, 2):
+        assert abs(p32[k] - p64[k]) <= 1e-6 * abs(p64[k])
+    SV = 8 ** 3
+    L32, L64 = f32.polyakov(), orc.polyakov(og, ref)
+    assert abs(L32 - L64) <= 1e-6 * max(abs(L64), SV ** -0.5)
+    for mode in (0, 1):
+        c32, c64 = f32.poly2(mode), orc.poly2(og, ref, mode)
+        scale = max(np.max(np.abs(c64)), 1e-300)
+        assert np.max(np.abs(c32 - c64)) <= 1e-6 * scale, (mode, np.max(np.abs(c32 - c64)) / scale)
+
+
+def test_fp32_overrelaxation_tracks_fp64(u1, orc):
+    """From a configuration exactly representable in FP32, one OR sweep in
+    FP32 stays within a few FP32 ulps of the FP64 sweep."""
+    dims = (8, 8, 8, 8)
+    g = u1.geom(dims)
+    og = orc.geom(dims)
+    start = orc.hotstart(og, 3).astype(np.float32).astype(np.float64)
+    a64 = u1.Lattice.from_host(g, start, u1.F64)
+    a32 = u1.Lattice.from_host(g, start, u1.F32)
+    a64.update(0, 1, u1.BASELINE_METRO, 3, 0)
+    a32.update(0, 1, u1.BASELINE_METRO, 3, 0)
+    d = np.angle(np.exp(1j * (a64.to_host() - a32.to_host().astype(np.float64))))
+    assert np.median(np.abs(d)) < 1e-5
+
+
+def test_fp32_chain_statistics_match_fp64(u1):
+    """Ensemble means: FP32 and FP64 chains (same Philox streams) agree within errors."""
+    dims = (8, 8, 8, 8)
+    g = u1.geom(dims, 1.0)
+    vals = {}
+    for prec in (u1.F64, u1.F32):
+        lat = u1.Lattice.cold(g, prec)
+        lat.hotstart(99)
+        v = []
+        for s in range(400):
+            lat.update(1, 3, u1.BASELINE_METRO, 99, s)
+            if s >= 100:
+                v.append(u1.mean_plaquette(lat.plaquette()))
+        vals[prec] = np.array(v)
+    a, b = vals[u1.F64], vals[u1.F32]
+    err = math.sqrt(a.var(ddof=1) / len(a) * 10 + b.var(ddof=1) / len(b) * 10)  # tau_int <~ 5
+    say(f"fp64 <P> = {a.mean():.6f}, fp32 <P> = {b.mean():.6f}, combined err ~ {err:.2e}")
+    assert abs(a.mean() - b.mean()) < 4 * err
+
+
+# ------------------------------------------------------------ batched replicas
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_fp64_matches_oracle(u1, orc, graph):
+    specs = [((4, 4, 4, 4), 0.9, 1.0, 1), ((6, 4, 4, 8), 1.05, 1.0, 0), ((8, 8, 8, 8), 1.1, 1.0, 1),
+             ((4, 4, 4, 8), 1.0, 2.0, 1)]
+    geoms = [u1.geom(d, b, x) for d, b, x, _ in specs]
+    seeds = [101 + i for i in range(len(specs))]
+    hot = [h for *_, h in specs]
+    e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BASELINE_METRO)
+    say(f"ensemble fp64 graph={graph}")
+    e.sweep(6, graph=graph)
+    p = e.plaquette()
+    for r, (d, b, x, h) in enumerate(specs):
+        og = orc.geom(d, b, x)
+        ref = orc.hotstart(og, seeds[r], replica=r) if h else orc.cold(og)
+        for s in range(6):
+            orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), seeds[r], s, c3=r)
+        assert np.array_equal(bits(e.read(r)), bits(ref)), f"replica {r}"
+        assert np.max(np.abs(p[r] - orc.plaquette(og, ref))) <= 1e-12
+    e.destroy()
+
+
+def test_ensemble_fp32_runs(u1):
+    geoms = [u1.geom((8, 8, 8, 8), 0.95), u1.geom((12, 12, 12, 16), 1.05)]
+    e = u1.Ensemble(geoms, [1, 2], [1, 0], u1.F32, 1, 3, u1.BASELINE_METRO)
+    e.sweep(20, graph=True)
+    p = e.plaquette()
+    assert np.all(np.isfinite(p))
+    assert 0.2 < u1.mean_plaquette(p[0]) < 0.8 and 0.2 < u1.mean_plaquette(p[1]) < 0.8
+    s, n = e.info()
+    assert s == 20 and n == 4 * (8 ** 4 + 12 ** 3 * 16)
+    e.destroy()
+
+
+def test_tuner_records_entries(u1, tmp_path):
+    p = tmp_path / "tune.csv"
+    u1.tune_save(str(p))
+    txt = p.read_text()
+    assert "Metropolis_f64" in txt and "OverRelaxation_f64" in txt
