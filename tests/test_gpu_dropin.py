@@ -1,0 +1,56 @@
+"""The C++ drop-in layer end to end: a reference-style driver (tests/dropin/
+u1_mini.cpp, built by build() with g++ against include/u1 and libu1.so) runs on
+the GPU; its lattice, observables and data files are checked against the oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "latticeu1_amd", "u1_mini")
+
+
+def test_dropin_driver_matches_oracle(orc, tmp_path):
+    assert os.path.exists(EXE), "build() has not produced latticeu1_amd/u1_mini"
+    print("[gpu] drop-in driver", flush=True)
+    out = subprocess.run([EXE, "20"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    vals = {}
+    for line in out.stdout.splitlines():
+        parts = line.split()
+        if parts and parts[0].isupper() and len(parts) > 1:
+            try:
+                vals[parts[0]] = [float(v) for v in parts[1:]]
+            except ValueError:
+                pass
+    dims = (8, 8, 8, 8)
+    og = orc.geom(dims, 1.0, 1.0)
+    m = orc.metro(1, 4, 0.5)
+    ref = orc.hotstart(og, 1234)
+    for s in range(20):
+        orc.update(og, ref, 1, 3, m, 1234, s)
+    lat = np.fromfile(tmp_path / "lattice.bin", dtype=np.float64)
+    assert np.array_equal(lat.view(np.uint64), ref.view(np.uint64))
+    p = orc.plaquette(og, ref)
+    assert np.allclose(vals["PLAQ"], p, atol=1e-12)
+    L = orc.polyakov(og, ref)
+    assert abs(complex(*vals["POLY"]) - L) < 1e-12
+    assert abs(complex(*vals["POLY2"]) - orc.poly2(og, ref, 0)[0]) < 1e-12
+    assert abs(complex(*vals["POLY2MH"]) - orc.poly2(og, ref, 1)[0]) < 1e-12
+    ml = orc.multilevel(og, ref, 2, 1, 2, 1, 1, 3, m, 1234, call_id=0, mh_mode=1)
+    assert abs(complex(*vals["ML"]) - ml[0]) < 1e-11 * max(1.0, abs(ml[0]))
+    assert vals["CONVERT_ROUNDTRIP"] == [1.0]
+    F, means = orc.plaquette_fields(og, ref, True, True)
+    assert abs(complex(*vals["FIELD0"]) - means[0]) < 1e-12
+    hp = orc.plaquette_host(og, ref)
+    assert np.allclose(vals["HOSTPLAQ"], hp, atol=1e-12)
+    assert 0.2 < 1 - 0.5 * (vals["PLAQ32"][0] + vals["PLAQ32"][1]) < 0.9
+    name = "8_8_8_8_beta_1_xi_1_mtr_1_ovr_3_iter_20"
+    for fn in (f"Pot_{name}.dat", f"Pot_mhit_{name}.dat", f"Pot_mlevel_{name}_2_1_2_1_1_3.dat"):
+        rows = (tmp_path / fn).read_text().strip().splitlines()
+        assert len(rows) == 4 and rows[0].split("\t")[0] == "1"
+    assert (tmp_path / "tunecache.csv").read_text().startswith("tunecache")
