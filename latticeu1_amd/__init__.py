@@ -394,7 +394,7 @@ def selftest_math(fn: int, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.
     """Evaluate a shared elementary function on the device (see u1_selftest_math)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y if y is not None else x, dtype=np.float64)
-    nout = 2 if fn in (0, 3, 5) else 1
+    nout = 2 if fn in (0, 3, 5, 7) else 1
     out = np.zeros(nout * x.size)
     call("u1_selftest_math", fn, _dp(x), _dp(y), _dp(out), x.size)
     return out.reshape(x.size, nout) if nout == 2 else out

@@ -79,6 +79,8 @@ int check_metro(const u1_metro* m) {
   if (m->proposal != U1_PROPOSAL_INDEPENDENT && m->proposal != U1_PROPOSAL_DELTA)
     return fail(U1_ERR_ARG, "unknown proposal kind");
   if (m->nhit < 1 || m->nhit > 8190) return fail(U1_ERR_ARG, "nhit must be in [1, 8190]");
+  if (m->proposal == U1_PROPOSAL_DELTA && !(m->step >= 0.0 && m->step <= 3.141592653589793))
+    return fail(U1_ERR_ARG, "the delta proposal needs 0 <= step <= pi");
   return U1_OK;
 }
 
@@ -110,6 +112,8 @@ RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
   return r;
 }
 
+void set_blocks(RepDesc& r, int block) { u1d::set_phase_blocks(r, block); }
+
 size_t elem_size(int prec) { return prec == U1_F64 ? sizeof(double) : sizeof(float); }
 
 int ensure_partials(size_t n) {
@@ -127,7 +131,7 @@ int ensure_partials(size_t n) {
 // (kernel class, geometry, precision), chosen by timing candidates with HIP
 // events on a scratch copy of the lattice, so tuning never perturbs the chain
 // (the reference backs up and restores lattice and RNG, tune.h preTune).
-const int kCandidates[] = {64, 128, 256};
+const int kCandidates[] = {64, 128, 192, 256, 384, 512};
 
 std::string vol_str(const u1_geom* g) {
   std::ostringstream s;
@@ -177,7 +181,8 @@ int phase_block(int prec, int kind, const u1_geom* g, const void* lat, const Pha
   for (int b : kCandidates) {
     PhaseArgs a = proto;
     a.accepts = nullptr;
-    const int nb = (proto.single.HV + b - 1) / b;
+    set_blocks(a.single, b);
+    const int nb = a.single.nblk;
     float ms = 0;
     int rc = time_phase(prec, kind, scratch, a, nb, b, &ms);
     if (rc != U1_OK) {
@@ -214,13 +219,17 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   int bm = 256, bo = 256;
   if (metrop > 0) RET(phase_block(prec, u1d::KIND_METROPOLIS, g, lat, a, &bm));
   if (ovrn > 0) RET(phase_block(prec, u1d::KIND_OVERRELAX, g, lat, a, &bo));
-  const int nbm = (a.single.HV + bm - 1) / bm, nbo = (a.single.HV + bo - 1) / bo;
+  RepDesc rm = a.single, ro = a.single;
+  set_blocks(rm, bm);
+  set_blocks(ro, bo);
+  const int nbm = rm.nblk, nbo = ro.nblk;
   int ev = 0;
   for (int p = 0; p < metrop; ++p)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)p;
         a.parity = parity;
+        a.single = rm;
         if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
         HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, nbm, bm, G.st));
         if (kev) {
@@ -234,6 +243,7 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
+        a.single = ro;
         if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
         HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, nbo, bo, G.st));
         if (kev) {
@@ -484,7 +494,8 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
   a.parity = parity;
   int b = 256;
   RET(phase_block(prec, kind, g, lat, a, &b));
-  HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, (a.single.HV + b - 1) / b, b, G.st));
+  set_blocks(a.single, b);
+  HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, b, G.st));
   return U1_OK;
 }
 
@@ -691,8 +702,8 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
     r.lat_off = (uint64_t)off;
     off += 4 * (int64_t)r.V;
     RepDesc u = r, q = r;
+    set_blocks(u, kEnsBlock);
     u.blk_begin = e->upd_blocks;
-    u.nblk = (r.HV + kEnsBlock - 1) / kEnsBlock;
     e->upd_blocks += u.nblk;
     for (int b = 0; b < u.nblk; ++b) umap.push_back((uint16_t)i);
     q.blk_begin = e->meas_blocks;
@@ -949,9 +960,9 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
 // ---------------------------------------------------------------- self test
 int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n) {
   REQ_INIT();
-  if (fn < 0 || fn > 6 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
+  if (fn < 0 || fn > 8 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
   if (n == 0) return U1_OK;
-  const int nout = (fn == 0 || fn == 3 || fn == 5) ? 2 : 1;
+  const int nout = (fn == 0 || fn == 3 || fn == 5 || fn == 7) ? 2 : 1;
   double *dx = nullptr, *dy = nullptr, *dout = nullptr;
   HIPCHK(hipMalloc((void**)&dx, n * sizeof(double)));
   HIPCHK(hipMalloc((void**)&dy, n * sizeof(double)));

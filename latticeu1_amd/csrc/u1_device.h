@@ -33,6 +33,8 @@ struct RepDesc {
   uint32_t c3;         // Philox counter word 3 (replica id)
   uint32_t div_nxh_m, div_ny_m, div_nz_m;  // magic multipliers for id/nxh, /Ny, /Nz
   uint32_t div_nxh_s, div_ny_s, div_nz_s;  // and their shifts
+  int32_t bpp;                             // update launches: workgroups per (z, t) plane
+  uint32_t div_bpp_m, div_bpp_s;           // magic division by bpp
   uint32_t pad0;
   double beta;
   double aniso;
@@ -87,6 +89,53 @@ __host__ __device__ __forceinline__ void site_of(const RepDesc& r, int32_t id, i
   s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
 }
 
+// Update-launch decomposition: workgroup b (relative to the replica's first)
+// covers chunk (b % bpp) of the (z, t) plane b / bpp; thread tid of a
+// workgroup of bdim threads takes even/odd site id = plane * P + chunk * bdim
+// + tid, P = Nx/2 * Ny.  z, t (and so the z/t neighbour offsets) are uniform
+// across the workgroup.  valid is false for the tail threads of a plane.
+struct PhaseSite {
+  bool valid;
+  uint32_t id, t;
+  int32_t z;
+  int32_t fwd[4], bwd[4];
+};
+
+__host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b, uint32_t tid, uint32_t bdim,
+                                                    int32_t parity, PhaseSite& s) {
+  const uint32_t plane = mdiv(b, r.div_bpp_m, r.div_bpp_s);
+  const uint32_t chunk = b - plane * (uint32_t)r.bpp;
+  const uint32_t t = mdiv(plane, r.div_nz_m, r.div_nz_s);
+  const int32_t z = (int32_t)(plane - t * (uint32_t)r.N[2]);
+  const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
+  const uint32_t i = chunk * bdim + tid;
+  s.valid = i < P;
+  s.t = t;
+  s.z = z;
+  const uint32_t y = mdiv(i, r.div_nxh_m, r.div_nxh_s);
+  const int32_t xh = (int32_t)(i - y * (uint32_t)r.nxh);
+  s.id = plane * P + i;
+  const int32_t xodd = ((int32_t)y + z + (int32_t)t + parity) & 1;
+  s.fwd[0] = xodd ? ((xh == r.nxh - 1) ? -(r.nxh - 1) : 1) : 0;
+  s.bwd[0] = xodd ? 0 : ((xh == 0) ? (r.nxh - 1) : -1);
+  s.fwd[1] = ((int32_t)y == r.N[1] - 1) ? -(r.N[1] - 1) * r.sy : r.sy;
+  s.bwd[1] = (y == 0) ? (r.N[1] - 1) * r.sy : -r.sy;
+  s.fwd[2] = (z == r.N[2] - 1) ? -(r.N[2] - 1) * r.sz : r.sz;
+  s.bwd[2] = (z == 0) ? (r.N[2] - 1) * r.sz : -r.sz;
+  s.fwd[3] = ((int32_t)t == r.N[3] - 1) ? -(r.N[3] - 1) * r.st : r.st;
+  s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
+}
+
+// Host helper: workgroup layout of update launches for a block size.
+inline void set_phase_blocks(RepDesc& r, int block) {
+  const int P = r.nxh * r.N[1];
+  r.bpp = (P + block - 1) / block;
+  const MagicDiv m = make_magic((uint32_t)r.bpp);
+  r.div_bpp_m = m.m;
+  r.div_bpp_s = m.s;
+  r.nblk = r.bpp * r.N[2] * r.N[3];
+}
+
 // ------------------------------------------------------- precision traits
 template <typename Real>
 struct Prec;
@@ -107,6 +156,45 @@ struct Prec<float> {
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
   static __device__ __forceinline__ float atan2(float y, float x) { return u1m::atan2_f(y, x); }
   static __device__ __forceinline__ float u01(uint32_t w) { return u1r::u01_f(w); }
+};
+
+// Update-kernel transcendentals: FP64 shares u1_math.h with the oracle; FP32
+// uses the hardware units (input of v_sin/v_cos in revolutions).
+template <typename Real>
+struct UMath;
+template <>
+struct UMath<double> {
+  static __device__ __forceinline__ void sincos(double x, double* s, double* c) { u1m::sincos_d(x, s, c); }
+  static __device__ __forceinline__ double exp(double x) { return u1m::exp_d(x); }
+  static __device__ __forceinline__ double atan2(double y, double x) { return u1m::atan2_d(y, x); }
+};
+template <>
+struct UMath<float> {
+  static __device__ __forceinline__ void sincos(float x, float* s, float* c) {
+    const float rev = x * 0x1.45f306p-3f;  // 1 / (2 pi)
+    *s = __builtin_amdgcn_sinf(rev);
+    *c = __builtin_amdgcn_cosf(rev);
+  }
+  static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+  // atan2 with the octant reduction of u1_math.h, divisions by v_rcp_f32
+  static __device__ __forceinline__ float atan2(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const bool swap = ay > ax;
+    const float num = swap ? ax : ay, den = swap ? ay : ax;
+    float z = den > 0.0f ? num * __builtin_amdgcn_rcpf(den) : 0.0f;
+    const bool red = z > u1m::KF_TAN_PIO8;
+    const float t = red ? (z - 1.0f) * __builtin_amdgcn_rcpf(z + 1.0f) : z;
+    const float t2 = t * t;
+    const float ca[u1m::N_ATAN32] = {U1M_ATAN32_COEFFS};
+    float p = ca[0];
+#pragma unroll
+    for (int i = 1; i < u1m::N_ATAN32; ++i) p = __builtin_fmaf(p, t2, ca[i]);
+    float a = __builtin_fmaf(t * t2, p, t);
+    if (red) a += u1m::KF_PIO4;
+    if (swap) a = (u1m::KF_PIO2_HI - a) + u1m::KF_PIO2_LO;
+    if (__builtin_signbit(x)) a = u1m::KF_PI - a;
+    return __builtin_signbit(y) ? -a : a;
+  }
 };
 
 // ------------------------------------------------------- wave64 reductions

@@ -35,7 +35,15 @@ __global__ __launch_bounds__(256) void k_math(int fn, const double* __restrict__
       out[2 * i + 1] = b;
       break;
     }
-    default: out[i] = u1m::wrap_d(x[i]); break;
+    case 6: out[i] = u1m::wrap_d(x[i]); break;
+    case 7: {
+      float s, c;
+      UMath<float>::sincos((float)x[i], &s, &c);
+      out[2 * i] = s;
+      out[2 * i + 1] = c;
+      break;
+    }
+    default: out[i] = UMath<float>::atan2((float)x[i], (float)y[i]); break;
   }
 }
 

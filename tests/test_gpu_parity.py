@@ -66,6 +66,20 @@ def test_device_math_bit_identical(u1, orc):
     assert np.max(np.abs(u1.selftest_math(4, xf, yf) - np.arctan2(xf, yf))) < 5e-7
 
 
+def test_fp32_update_math_accuracy(u1):
+    """Hardware transcendentals used by the FP32 update kernels (not by the
+    measurement kernels, which keep the polynomial sincos_f)."""
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-3 * np.pi, 3 * np.pi, 50000).astype(np.float32).astype(np.float64)
+    f = u1.selftest_math(7, x)
+    es = np.max(np.abs(f[:, 0] - np.sin(x)))
+    ec = np.max(np.abs(f[:, 1] - np.cos(x)))
+    y = rng.uniform(-6, 6, 50000).astype(np.float32).astype(np.float64)
+    ea = np.max(np.abs(u1.selftest_math(8, x, y) - np.arctan2(x, y)))
+    say(f"fp32 update math: max |err| sin {es:.2e} cos {ec:.2e} atan2 {ea:.2e}")
+    assert es < 5e-6 and ec < 5e-6 and ea < 2e-6
+
+
 def test_device_philox_kat(u1):
     from tests.test_oracle_math import KAT
 
