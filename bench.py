@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--calib-mb", type=float, default=0.0,
+                    help="after the run, copy a buffer of this many MiB once (plain copy kernel of known size, "
+                         "used to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -151,6 +154,13 @@ def main():
         p32 = f64["lat"].astype(u1.F32).plaquette()
         extra["fixed_config_plaquette_rel_err"] = float(abs(u1.mean_plaquette(p32) - u1.mean_plaquette(p64))
                                                         / abs(u1.mean_plaquette(p64)))
+
+    if args.calib_mb > 0:
+        n = int(args.calib_mb * 1048576 / 4)
+        src, dst = u1.DeviceArray(n, np.float32), u1.DeviceArray(n, np.float32)
+        u1.call("u1_convert_precision", src.ptr, u1.F32, dst.ptr, u1.F32, n)
+        u1.sync()
+        print(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.skip_cpu:

@@ -146,6 +146,17 @@ int u1_convert_layout(const u1_geom* g, int prec, const void* in, void* out, int
 int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spacetime, int32_t eo_order,
                         void* field_dev, double means[12]);
 
+/* ---------------------------------------------------------------- checkpoint */
+/* Configuration file (no counterpart in the reference, which saves no
+ * configurations): a 64-byte little-endian header -- magic "U1AMDCFG",
+ * version 1, dims[4], beta, xi, seed, sweep -- then 4V binary64 angles in the
+ * lexicographic order of LatticeConvert(eo_to_no) ([mu][lex]).  With the
+ * counter-based generator, (configuration, seed, sweep) resumes a chain
+ * exactly.  Load checks the file's geometry against g; seed/sweep (nullable)
+ * receive the stored counters. */
+int u1_config_save(const char* path, const u1_geom* g, int prec, const void* lat, uint64_t seed, uint64_t sweep);
+int u1_config_load(const char* path, const u1_geom* g, int prec, void* lat, uint64_t* seed, uint64_t* sweep);
+
 /* ---------------------------------------------------------------- batched replicas */
 typedef struct u1_ensemble u1_ensemble;
 /* nrep independent lattices (own geometry, beta, xi, seed; hot[r] != 0 for a hot
@@ -173,11 +184,12 @@ int u1_tune_save(const char* path);
 int u1_tune_entries(int32_t* n);
 
 /* ---------------------------------------------------------------- timing helper */
-/* Run nsweeps of u1_update (sweep indices sweep0, sweep0+1, ...) and return the
- * elapsed device time in ms measured with HIP events on the library stream, and
- * the mean duration of each phase kernel class (Metropolis, OR) if kernel_ms != NULL
- * (kernel_ms[0] = mean Metropolis phase, kernel_ms[1] = mean OR phase, measured
- * in a separate event-bracketed pass over the same number of sweeps). */
+/* Run nsweeps of u1_update (sweep indices sweep0, sweep0+1, ...; a plaquette
+ * measurement into plaq_series[4*i] after every sweep if measure_plaq) and
+ * return the elapsed device time in ms, from HIP events on the library stream.
+ * kernel_ms (nullable): mean launch duration of the Metropolis (kernel_ms[0])
+ * and overrelaxation (kernel_ms[1]) phase kernels, from events that bracket
+ * each sweep's back-to-back launch groups inside the same timed region. */
 int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms);

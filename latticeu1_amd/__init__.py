@@ -94,6 +94,8 @@ _SIGS = {
                       _P),
     "u1_convert_layout": (C.POINTER(Geom), C.c_int, _P, _P, C.c_int32),
     "u1_plaquette_fields": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.c_int32, _P, C.POINTER(C.c_double)),
+    "u1_config_save": (C.c_char_p, C.POINTER(Geom), C.c_int, _P, C.c_uint64, C.c_uint64),
+    "u1_config_load": (C.c_char_p, C.POINTER(Geom), C.c_int, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)),
     "u1_ensemble_create": (C.POINTER(_P), C.c_int32, C.POINTER(Geom), C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
                            C.c_int, C.c_int32, C.c_int32, C.POINTER(Metro)),
     "u1_ensemble_sweep": (_P, C.c_int32, C.c_int32),
@@ -310,6 +312,15 @@ class Lattice:
             h = pps.to_host()
             return res, h[0::2] + 1j * h[1::2]
         return res
+
+    # -- configuration files
+    def save(self, path: str, seed: int, sweep: int) -> None:
+        call("u1_config_save", path.encode(), C.byref(self.g), self.prec, self.buf.ptr, seed, sweep)
+
+    def load(self, path: str):
+        seed, sweep = C.c_uint64(0), C.c_uint64(0)
+        call("u1_config_load", path.encode(), C.byref(self.g), self.prec, self.buf.ptr, C.byref(seed), C.byref(sweep))
+        return int(seed.value), int(sweep.value)
 
     # -- array.h LatticeConvert
     def convert_layout(self, eo_to_no: bool) -> "Lattice":

@@ -31,7 +31,7 @@ struct Ctx {
   unsigned long long* counter = nullptr;
   double* dres = nullptr;  // small device result buffer (64 doubles)
   bool tune = true;
-  std::map<std::string, int> tunecache;
+  std::map<std::string, std::pair<int, int>> tunecache;  // key -> (block, sites per thread)
   int tune_loaded = 0;
 };
 Ctx G;
@@ -127,11 +127,18 @@ int ensure_partials(size_t n) {
 }
 
 // ------------------------------------------------------------------ autotuner
-// Own tuner (replacing the QUDA-derived tune.cpp:306-416): workgroup size per
-// (kernel class, geometry, precision), chosen by timing candidates with HIP
-// events on a scratch copy of the lattice, so tuning never perturbs the chain
-// (the reference backs up and restores lattice and RNG, tune.h preTune).
-const int kCandidates[] = {64, 128, 192, 256, 384, 512};
+// Own tuner (replacing the QUDA-derived tune.cpp:306-416): launch shape per
+// (kernel class, geometry, precision) -- workgroup size and even/odd sites per
+// thread -- chosen by timing every candidate with HIP events on a scratch copy
+// of the lattice, so tuning never perturbs the chain (the reference backs up
+// and restores lattice and RNG instead, tune.h preTune/postTune).  Results do
+// not depend on the launch shape.
+struct Cfg {
+  int block, spt;
+};
+const Cfg kCandidates[] = {{64, 1}, {128, 1}, {192, 1}, {256, 1}, {384, 1}, {512, 1}, {64, 2},
+                           {128, 2}, {192, 2}, {256, 2}, {384, 2}, {64, 4},  {128, 4}, {192, 4}, {256, 4}};
+const Cfg kDefaultCfg = {256, 1};
 
 std::string vol_str(const u1_geom* g) {
   std::ostringstream s;
@@ -142,15 +149,14 @@ std::string tune_key(const char* name, const u1_geom* g, int prec) {
   return vol_str(g) + "\t" + name + (prec == U1_F64 ? "_f64" : "_f32");
 }
 
-int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, int block, float* ms) {
+int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, Cfg c, float* ms) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  // warm-up
-  HIPCHK(u1d::launch_phase(prec, kind, 0, lat, a, nb, block, G.st));
+  HIPCHK(u1d::launch_phase(prec, kind, 0, lat, a, nb, c.block, c.spt, G.st));  // warm-up
   HIPCHK(hipEventRecord(e0, G.st));
   for (int rep = 0; rep < 2; ++rep)
-    for (int mu = 0; mu < 4; ++mu) HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, nb, block, G.st));
+    for (int mu = 0; mu < 4; ++mu) HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, nb, c.block, c.spt, G.st));
   HIPCHK(hipEventRecord(e1, G.st));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
@@ -159,17 +165,17 @@ int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, int bl
   return U1_OK;
 }
 
-// Block size for the update phase kernel class `kind` on this geometry.
-int phase_block(int prec, int kind, const u1_geom* g, const void* lat, const PhaseArgs& proto, int* block) {
+// Launch shape for the update phase kernel class `kind` on this geometry.
+int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const PhaseArgs& proto, Cfg* cfg) {
   const char* name = kind == u1d::KIND_METROPOLIS ? "Metropolis" : "OverRelaxation";
   const std::string key = tune_key(name, g, prec);
   auto it = G.tunecache.find(key);
   if (it != G.tunecache.end()) {
-    *block = it->second;
+    *cfg = {it->second.first, it->second.second};
     return U1_OK;
   }
   if (!G.tune) {
-    *block = 256;
+    *cfg = kDefaultCfg;
     return U1_OK;
   }
   const int64_t bytes = 4 * (int64_t)proto.single.V * (int64_t)elem_size(prec);
@@ -177,33 +183,39 @@ int phase_block(int prec, int kind, const u1_geom* g, const void* lat, const Pha
   HIPCHK(hipMalloc(&scratch, bytes));
   HIPCHK(hipMemcpyAsync(scratch, lat, bytes, hipMemcpyDeviceToDevice, G.st));
   float best = 1e30f;
-  int bestb = 256;
-  for (int b : kCandidates) {
+  Cfg bestc = kDefaultCfg;
+  for (const Cfg c : kCandidates) {
+    if (c.spt == 4 && prec != U1_F32) continue;
     PhaseArgs a = proto;
     a.accepts = nullptr;
-    set_blocks(a.single, b);
-    const int nb = a.single.nblk;
+    set_blocks(a.single, c.block * c.spt);
     float ms = 0;
-    int rc = time_phase(prec, kind, scratch, a, nb, b, &ms);
+    int rc = time_phase(prec, kind, scratch, a, a.single.nblk, c, &ms);
     if (rc != U1_OK) {
       (void)hipFree(scratch);
       return rc;
     }
     if (ms < best) {
       best = ms;
-      bestb = b;
+      bestc = c;
     }
   }
   HIPCHK(hipFree(scratch));
-  G.tunecache[key] = bestb;
-  *block = bestb;
+  G.tunecache[key] = {bestc.block, bestc.spt};
+  *cfg = bestc;
   return U1_OK;
 }
+
+// Events bracketing the Metropolis and overrelaxation groups of one sweep
+// (for the mean launch duration of each kernel class).
+struct GroupEvents {
+  hipEvent_t start, after_metro, after_or;
+};
 
 // One UpdateLattice on a single lattice (update.cu:449-456 loop order).
 int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, const u1_metro* m, uint64_t seed,
                uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
-               hipEvent_t* kev = nullptr, std::vector<std::pair<int, int>>* kev_kind = nullptr) {
+               const GroupEvents* gev = nullptr) {
   PhaseArgs a{};
   a.single = make_rep(g, seed, c3);
   a.reps = nullptr;
@@ -216,42 +228,31 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   a.sweep = sweep;
   a.sweep_dev = nullptr;
   a.accepts = acc_dev;
-  int bm = 256, bo = 256;
-  if (metrop > 0) RET(phase_block(prec, u1d::KIND_METROPOLIS, g, lat, a, &bm));
-  if (ovrn > 0) RET(phase_block(prec, u1d::KIND_OVERRELAX, g, lat, a, &bo));
+  Cfg cm = kDefaultCfg, co = kDefaultCfg;
+  if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
+  if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &co));
   RepDesc rm = a.single, ro = a.single;
-  set_blocks(rm, bm);
-  set_blocks(ro, bo);
-  const int nbm = rm.nblk, nbo = ro.nblk;
-  int ev = 0;
+  set_blocks(rm, cm.block * cm.spt);
+  set_blocks(ro, co.block * co.spt);
+  if (gev) HIPCHK(hipEventRecord(gev->start, G.st));
+  a.single = rm;
   for (int p = 0; p < metrop; ++p)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)p;
         a.parity = parity;
-        a.single = rm;
-        if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
-        HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, nbm, bm, G.st));
-        if (kev) {
-          HIPCHK(hipEventRecord(kev[2 * ev + 1], G.st));
-          kev_kind->push_back({ev, 0});
-          ++ev;
-        }
+        HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, rm.nblk, cm.block, cm.spt, G.st));
       }
+  if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
+  a.single = ro;
   for (int o = 0; o < ovrn; ++o)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
-        a.single = ro;
-        if (kev) HIPCHK(hipEventRecord(kev[2 * ev], G.st));
-        HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, nbo, bo, G.st));
-        if (kev) {
-          HIPCHK(hipEventRecord(kev[2 * ev + 1], G.st));
-          kev_kind->push_back({ev, 1});
-          ++ev;
-        }
+        HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, ro.nblk, co.block, co.spt, G.st));
       }
+  if (gev) HIPCHK(hipEventRecord(gev->after_or, G.st));
   return U1_OK;
 }
 
@@ -294,13 +295,38 @@ struct u1_ensemble {
   double* partials = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  Cfg cfg = {256, 1};  // launch shape of the update phases (tuned on first use)
+  bool tuned = false;
 };
 
 namespace {
 
-const int kEnsBlock = 256;
+const int kEnsBlock = 256;  // measurement launches
 
-int ensemble_enqueue_sweep(u1_ensemble* e) {
+// (Re)build the update-launch tables for launch shape c: per-replica plane
+// decomposition and first workgroup, and the workgroup -> replica map.
+int ensemble_build_update(u1_ensemble* e, Cfg c) {
+  std::vector<uint16_t> umap;
+  int nb = 0;
+  for (int i = 0; i < e->nrep; ++i) {
+    RepDesc& u = e->upd_host[i];
+    set_blocks(u, c.block * c.spt);
+    u.blk_begin = nb;
+    nb += u.nblk;
+    for (int b = 0; b < u.nblk; ++b) umap.push_back((uint16_t)i);
+  }
+  if (e->upd_map) HIPCHK(hipFree(e->upd_map));
+  e->upd_map = nullptr;
+  HIPCHK(hipMalloc((void**)&e->upd_map, umap.size() * sizeof(uint16_t)));
+  HIPCHK(hipMemcpyAsync(e->upd_map, umap.data(), umap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(e->upd_dev, e->upd_host.data(), e->nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  e->upd_blocks = nb;
+  e->cfg = c;
+  return U1_OK;
+}
+
+int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
   PhaseArgs a{};
   a.reps = e->upd_dev;
   a.blockmap = e->upd_map;
@@ -317,16 +343,58 @@ int ensemble_enqueue_sweep(u1_ensemble* e) {
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)p;
         a.parity = parity;
-        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_METROPOLIS, mu, e->lat, a, e->upd_blocks, kEnsBlock, G.st));
+        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_METROPOLIS, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
+                                 G.st));
       }
   for (int o = 0; o < e->ovrn; ++o)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
-        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_OVERRELAX, mu, e->lat, a, e->upd_blocks, kEnsBlock, G.st));
+        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_OVERRELAX, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
+                                 G.st));
       }
+  return U1_OK;
+}
+
+int ensemble_enqueue_sweep(u1_ensemble* e) {
+  RET(ensemble_enqueue_phases(e, e->lat));
   HIPCHK(u1d::launch_increment(e->sweep_dev, G.st));
+  return U1_OK;
+}
+
+// Time every launch-shape candidate over one full sweep of all replicas on a
+// scratch copy of the packed lattice; keep the fastest.
+int ensemble_tune(u1_ensemble* e) {
+  const size_t bytes = (size_t)e->total_links * elem_size(e->prec);
+  void* scratch = nullptr;
+  HIPCHK(hipMalloc(&scratch, bytes));
+  HIPCHK(hipMemcpyAsync(scratch, e->lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  Cfg bestc = e->cfg;
+  for (const Cfg c : kCandidates) {
+    if (c.spt == 4 && e->prec != U1_F32) continue;
+    RET(ensemble_build_update(e, c));
+    RET(ensemble_enqueue_phases(e, scratch));  // warm-up
+    HIPCHK(hipEventRecord(e0, G.st));
+    RET(ensemble_enqueue_phases(e, scratch));
+    HIPCHK(hipEventRecord(e1, G.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestc = c;
+    }
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  HIPCHK(hipFree(scratch));
+  RET(ensemble_build_update(e, bestc));
+  e->tuned = true;
   return U1_OK;
 }
 
@@ -492,10 +560,10 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
   a.sweep = sweep;
   a.pass = pass;
   a.parity = parity;
-  int b = 256;
-  RET(phase_block(prec, kind, g, lat, a, &b));
-  set_blocks(a.single, b);
-  HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, b, G.st));
+  Cfg c = kDefaultCfg;
+  RET(phase_cfg(prec, kind, g, lat, a, &c));
+  set_blocks(a.single, c.block * c.spt);
+  HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, c.block, c.spt, G.st));
   return U1_OK;
 }
 
@@ -675,6 +743,84 @@ int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spa
   return copy_back(means, G.dres, 12);
 }
 
+// ---------------------------------------------------------------- checkpoint
+namespace {
+struct CfgHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t dims[4];
+  uint32_t reserved;
+  double beta, aniso;
+  uint64_t seed, sweep;
+};
+static_assert(sizeof(CfgHeader) == 64, "config header must be 64 bytes");
+}  // namespace
+
+int u1_config_save(const char* path, const u1_geom* g, int prec, const void* lat, uint64_t seed, uint64_t sweep) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  if (!path || !lat) return fail(U1_ERR_ARG, "null argument");
+  const RepDesc r = make_rep(g, 0, 0);
+  const size_t n = 4 * (size_t)r.V;
+  void *no = nullptr, *d64 = nullptr;
+  HIPCHK(hipMalloc(&no, n * elem_size(prec)));
+  HIPCHK(hipMalloc(&d64, n * sizeof(double)));
+  HIPCHK(u1d::launch_convert_layout(prec, lat, no, r, 1, 256, G.st));
+  HIPCHK(u1d::launch_convert_precision(no, prec, d64, U1_F64, (int64_t)n, 256, G.st));
+  std::vector<double> host(n);
+  HIPCHK(hipMemcpyAsync(host.data(), d64, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(no));
+  HIPCHK(hipFree(d64));
+  CfgHeader h{};
+  std::memcpy(h.magic, "U1AMDCFG", 8);
+  h.version = 1;
+  for (int i = 0; i < 4; ++i) h.dims[i] = g->dims[i];
+  h.beta = g->beta;
+  h.aniso = g->aniso;
+  h.seed = seed;
+  h.sweep = sweep;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return fail(U1_ERR_IO, std::string("cannot write ") + path);
+  f.write((const char*)&h, sizeof(h));
+  f.write((const char*)host.data(), n * sizeof(double));
+  return f ? U1_OK : fail(U1_ERR_IO, std::string("write failed: ") + path);
+}
+
+int u1_config_load(const char* path, const u1_geom* g, int prec, void* lat, uint64_t* seed, uint64_t* sweep) {
+  if (!path || !g) return fail(U1_ERR_ARG, "null argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(U1_ERR_IO, std::string("cannot open ") + path);
+  CfgHeader h{};
+  f.read((char*)&h, sizeof(h));
+  if (!f || std::memcmp(h.magic, "U1AMDCFG", 8) != 0 || h.version != 1)
+    return fail(U1_ERR_IO, std::string("not a configuration file: ") + path);
+  for (int i = 0; i < 4; ++i)
+    if (h.dims[i] != g->dims[i]) return fail(U1_ERR_ARG, "configuration file geometry differs from the lattice");
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  if (!lat) return fail(U1_ERR_ARG, "null lattice");
+  const RepDesc r = make_rep(g, 0, 0);
+  const size_t n = 4 * (size_t)r.V;
+  std::vector<double> host(n);
+  f.read((char*)host.data(), n * sizeof(double));
+  if (!f) return fail(U1_ERR_IO, std::string("truncated configuration file: ") + path);
+  void *no = nullptr, *d64 = nullptr;
+  HIPCHK(hipMalloc(&no, n * elem_size(prec)));
+  HIPCHK(hipMalloc(&d64, n * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d64, host.data(), n * sizeof(double), hipMemcpyHostToDevice, G.st));
+  HIPCHK(u1d::launch_convert_precision(d64, U1_F64, no, prec, (int64_t)n, 256, G.st));
+  HIPCHK(u1d::launch_convert_layout(prec, no, lat, r, 0, 256, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(no));
+  HIPCHK(hipFree(d64));
+  if (seed) *seed = h.seed;
+  if (sweep) *sweep = h.sweep;
+  return U1_OK;
+}
+
 // ---------------------------------------------------------------- ensemble
 int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, const uint64_t* seeds,
                        const int32_t* hot, int prec, int32_t metrop, int32_t ovrn, const u1_metro* m) {
@@ -689,7 +835,7 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
   e->metrop = metrop;
   e->ovrn = ovrn;
   e->m = *m;
-  std::vector<uint16_t> umap, mmap;
+  std::vector<uint16_t> mmap;
   int64_t off = 0;
   for (int i = 0; i < nrep; ++i) {
     int rc = check_geom(&geoms[i]);
@@ -702,10 +848,6 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
     r.lat_off = (uint64_t)off;
     off += 4 * (int64_t)r.V;
     RepDesc u = r, q = r;
-    set_blocks(u, kEnsBlock);
-    u.blk_begin = e->upd_blocks;
-    e->upd_blocks += u.nblk;
-    for (int b = 0; b < u.nblk; ++b) umap.push_back((uint16_t)i);
     q.blk_begin = e->meas_blocks;
     q.nblk = (r.V + kEnsBlock - 1) / kEnsBlock;
     e->meas_blocks += q.nblk;
@@ -728,16 +870,18 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
   ECHK(u1d::launch_fill_zero(e->lat, off * (int64_t)elem_size(prec), G.st));
   ECHK(hipMalloc((void**)&e->upd_dev, nrep * sizeof(RepDesc)));
   ECHK(hipMalloc((void**)&e->meas_dev, nrep * sizeof(RepDesc)));
-  ECHK(hipMalloc((void**)&e->upd_map, umap.size() * sizeof(uint16_t)));
   ECHK(hipMalloc((void**)&e->meas_map, mmap.size() * sizeof(uint16_t)));
   ECHK(hipMalloc((void**)&e->sweep_dev, sizeof(uint32_t)));
   ECHK(hipMalloc((void**)&e->res_dev, 4 * (size_t)nrep * sizeof(double)));
   ECHK(hipMalloc((void**)&e->partials, 4 * (size_t)e->meas_blocks * sizeof(double)));
-  ECHK(hipMemcpyAsync(e->upd_dev, e->upd_host.data(), nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
   ECHK(hipMemcpyAsync(e->meas_dev, e->meas_host.data(), nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
-  ECHK(hipMemcpyAsync(e->upd_map, umap.data(), umap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
   ECHK(hipMemcpyAsync(e->meas_map, mmap.data(), mmap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
   ECHK(u1d::launch_fill_zero(e->sweep_dev, sizeof(uint32_t), G.st));
+  if (int rc = ensemble_build_update(e, e->cfg); rc != U1_OK) {
+    ensemble_release(e);
+    delete e;
+    return rc;
+  }
   for (int i = 0; i < nrep; ++i)
     if (hot && hot[i]) ECHK(u1d::launch_hotstart(prec, e->lat, e->upd_host[i], 256, G.st));
   ECHK(hipStreamSynchronize(G.st));
@@ -749,6 +893,7 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
 int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
   REQ_INIT();
   if (!e) return fail(U1_ERR_ARG, "null ensemble");
+  if (G.tune && !e->tuned && !e->exec) RET(ensemble_tune(e));
   if (use_graph && !e->exec) {
     HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
     int rc = ensemble_enqueue_sweep(e);
@@ -862,7 +1007,7 @@ int u1_tune_load(const char* path) {
     if (!(std::getline(ls, vol, '\t') && std::getline(ls, name, '\t') && std::getline(ls, aux, '\t')))
       return fail(U1_ERR_IO, std::string("Bad format in ") + path);
     if (!(ls >> bx >> by >> bz >> gx >> gy >> gz >> sh)) return fail(U1_ERR_IO, std::string("Bad format in ") + path);
-    G.tunecache[vol + "\t" + name] = bx;
+    G.tunecache[vol + "\t" + name] = {bx, (by == 2 || by == 4) ? by : 1};
     ++n;
   }
   G.tune_loaded = n;
@@ -879,7 +1024,8 @@ int u1_tune_save(const char* path) {
     const std::string& key = kv.first;
     const size_t tab = key.find('\t');
     const std::string vol = key.substr(0, tab), name = key.substr(tab + 1);
-    f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second << "\t1\t1\t0\t1\t1\t0\t# workgroup size\n";
+    f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.first << "\t" << kv.second.second
+      << "\t1\t0\t1\t1\t0\t# block.x = workgroup size, block.y = even/odd sites per thread\n";
   }
   return f ? U1_OK : fail(U1_ERR_IO, "write failed");
 }
@@ -907,9 +1053,17 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
     a.nhit = m->nhit;
     a.proposal = m->proposal;
     a.step = m->step;
-    int b;
-    if (metrop > 0) RET(phase_block(prec, u1d::KIND_METROPOLIS, g, lat, a, &b));
-    if (ovrn > 0) RET(phase_block(prec, u1d::KIND_OVERRELAX, g, lat, a, &b));
+    Cfg c;
+    if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &c));
+    if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &c));
+  }
+  // Timed region: every sweep brackets its Metropolis and OR launch groups
+  // with events (kernel_ms), plus one event pair around the whole region.
+  std::vector<GroupEvents> gev(kernel_ms ? (size_t)nsweeps : 0);
+  for (auto& e : gev) {
+    HIPCHK(hipEventCreate(&e.start));
+    HIPCHK(hipEventCreate(&e.after_metro));
+    HIPCHK(hipEventCreate(&e.after_or));
   }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
@@ -917,7 +1071,8 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   HIPCHK(hipStreamSynchronize(G.st));
   HIPCHK(hipEventRecord(e0, G.st));
   for (int s = 0; s < nsweeps; ++s) {
-    RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)s, 0, 0, nullptr));
+    RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)s, 0, 0, nullptr,
+                   kernel_ms ? &gev[s] : nullptr));
     if (measure_plaq) RET(plaquette_async(g, prec, lat, series + 4 * s));
   }
   HIPCHK(hipEventRecord(e1, G.st));
@@ -925,33 +1080,24 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *elapsed_ms = ms;
+  if (kernel_ms) {
+    double tm = 0, to = 0;
+    for (auto& e : gev) {
+      float a = 0, b = 0;
+      HIPCHK(hipEventElapsedTime(&a, e.start, e.after_metro));
+      HIPCHK(hipEventElapsedTime(&b, e.after_metro, e.after_or));
+      tm += a;
+      to += b;
+      HIPCHK(hipEventDestroy(e.start));
+      HIPCHK(hipEventDestroy(e.after_metro));
+      HIPCHK(hipEventDestroy(e.after_or));
+    }
+    kernel_ms[0] = metrop > 0 ? tm / (8.0 * metrop * nsweeps) : 0.0;
+    kernel_ms[1] = ovrn > 0 ? to / (8.0 * ovrn * nsweeps) : 0.0;
+  }
   if (measure_plaq && plaq_series)
     HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));
   if (series) HIPCHK(hipFree(series));
-  if (kernel_ms) {
-    // per-kernel durations: start/stop events around every phase launch
-    const int nk = 8 * (metrop + ovrn);
-    const int ns = std::min(nsweeps, 16);
-    std::vector<hipEvent_t> ev(2 * (size_t)nk);
-    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
-    double sum[2] = {0, 0};
-    long cnt[2] = {0, 0};
-    for (int s = 0; s < ns; ++s) {
-      std::vector<std::pair<int, int>> kinds;
-      RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)(nsweeps + s), 0, 0,
-                     nullptr, ev.data(), &kinds));
-      HIPCHK(hipEventSynchronize(ev.back()));
-      for (auto& kd : kinds) {
-        float t = 0;
-        HIPCHK(hipEventElapsedTime(&t, ev[2 * kd.first], ev[2 * kd.first + 1]));
-        sum[kd.second] += t;
-        cnt[kd.second] += 1;
-      }
-    }
-    for (auto& x : ev) HIPCHK(hipEventDestroy(x));
-    kernel_ms[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;
-    kernel_ms[1] = cnt[1] ? sum[1] / cnt[1] : 0.0;
-  }
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   return U1_OK;

@@ -28,8 +28,10 @@ struct PhaseArgs {
 enum { KIND_METROPOLIS = 0, KIND_OVERRELAX = 1 };
 enum { PREC_F64 = 0, PREC_F32 = 1 };
 
+// spt: even/odd sites per thread (1 or 2); the replica descriptors' plane
+// decomposition must have been made with set_phase_blocks(r, block * spt).
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
-                        hipStream_t st);
+                        int spt, hipStream_t st);
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t st);
 

@@ -115,50 +115,79 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
   }
 }
 
-template <typename Real, int MU, int KIND, bool BATCHED>
+// SPT even/odd sites per thread: sites chunk*SPT*bdim + k*bdim + tid, k < SPT,
+// of one (z, t) plane.  All angle loads of all SPT sites are issued before the
+// transcendental work, so each wave keeps SPT x 19 loads in flight.  Threads
+// past the end of a plane are pointed at site 0 (in bounds) and do not store.
+template <typename Real, int MU, int KIND, bool BATCHED, int SPT>
 __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, PhaseArgs a) {
   const RepDesc& r = BATCHED ? a.reps[a.blockmap[blockIdx.x]] : a.single;
-  PhaseSite ps;
-  phase_site(r, blockIdx.x - (uint32_t)r.blk_begin, threadIdx.x, blockDim.x, a.parity, ps);
-  bool active = ps.valid;
-  // multilevel.cu:41,141 -- spatial links on slices t % ml == 0 are frozen
-  if (MU != 3 && a.ml != 0 && (ps.t & (uint32_t)(a.ml - 1)) == 0) active = false;
+  const uint32_t b = blockIdx.x - (uint32_t)r.blk_begin;
+  PhaseSite ps[SPT];
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, a.parity, ps[k]);
+  // multilevel.cu:41,141 -- spatial links on slices t % ml == 0 are frozen;
+  // t is uniform over the workgroup, so the whole workgroup leaves.
+  if (MU != 3 && a.ml != 0 && (ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
   Real* lat = lat_base + r.lat_off;
-  int acc = 0;
-  if (active) {
-    const int32_t (&fwd)[4] = ps.fwd;
-    const int32_t (&bwd)[4] = ps.bwd;
-    const uint32_t same = (uint32_t)(a.parity * r.HV) + ps.id;
-    const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps.id;
-    const bool aniso = r.aniso != 1.0;
-    const Staple<Real> S =
-        staple_at<Real, MU>(lat, r, same, opp, fwd, bwd, (Real)r.aniso, (Real)(1.0 / r.aniso), aniso);
-    Real* pmu = lat + (size_t)MU * r.V;
-    Real th = ld(pmu, same);
-    if (KIND == KIND_METROPOLIS) {
-      MetroState<Real> m;
-      m.th = th;
-      Real s0, c0;
-      UMath<Real>::sincos(th, &s0, &c0);
-      m.re_old = S.re * c0 - S.im * s0;
-      m.acc = 0;
-      const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
-      const Real beta = (Real)r.beta;
-      const uint32_t l = same + (uint32_t)MU * (uint32_t)r.V;
-      for (int h = 0; h < a.nhit; h += 2) {
-        const u1r::U32x4 w = u1r::draw(r.seed, l, sweep, u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1)), r.c3);
-        metro_hit<Real>(m, S, beta, a.proposal, a.step, w.v[0], w.v[1]);
-        if (h + 1 < a.nhit) metro_hit<Real>(m, S, beta, a.proposal, a.step, w.v[2], w.v[3]);
-      }
-      th = m.th;
-      acc = m.acc;
-    } else {
-      // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped in FP64
-      const Real t2 = UMath<Real>::atan2(S.im, S.re);
-      th = (Real)u1m::wrap_d(-(double)th - 2.0 * (double)t2);
+  Real* pmu = lat + (size_t)MU * r.V;
+  const bool aniso = r.aniso != 1.0;
+  uint32_t same[SPT];
+  Staple<Real> S[SPT];
+  Real th[SPT];
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    if (!ps[k].valid) {
+      ps[k].id = 0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) ps[k].fwd[d] = ps[k].bwd[d] = 0;
     }
-    st(pmu, same, th);
+    same[k] = (uint32_t)(a.parity * r.HV) + ps[k].id;
+    const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
+    S[k] = staple_at<Real, MU>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, (Real)r.aniso, (Real)(1.0 / r.aniso),
+                               aniso);
+    th[k] = ld(pmu, same[k]);
   }
+  int acc = 0;
+  if (KIND == KIND_METROPOLIS) {
+    MetroState<Real> m[SPT];
+    uint32_t l[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      m[k].th = th[k];
+      Real s0, c0;
+      UMath<Real>::sincos(th[k], &s0, &c0);
+      m[k].re_old = S[k].re * c0 - S[k].im * s0;
+      m[k].acc = 0;
+      l[k] = same[k] + (uint32_t)MU * (uint32_t)r.V;
+    }
+    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+    const Real beta = (Real)r.beta;
+    for (int h = 0; h < a.nhit; h += 2) {
+      const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) {
+        const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
+        metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
+        if (h + 1 < a.nhit) metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      th[k] = m[k].th;
+      if (ps[k].valid) acc += m[k].acc;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped in FP64
+      const Real t2 = UMath<Real>::atan2(S[k].im, S[k].re);
+      th[k] = (Real)u1m::wrap_d(-(double)th[k] - 2.0 * (double)t2);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < SPT; ++k)
+    if (ps[k].valid) st(pmu, same[k], th[k]);
   if (KIND == KIND_METROPOLIS && a.accepts) {
     const uint64_t tot = wave_sum_u64((uint64_t)acc);
     if ((threadIdx.x & 63) == 0 && tot) atomicAdd(a.accepts, (unsigned long long)tot);
@@ -173,33 +202,45 @@ __global__ __launch_bounds__(256) void k_hotstart(Real* __restrict__ lat, RepDes
   lat[l] = (Real)(u1m::K_PI * (2.0 * u1r::u01_d(w.v[0]) - 1.0));
 }
 
-template <typename Real, int KIND, bool BATCHED>
+template <typename Real, int KIND, bool BATCHED, int SPT>
 hipError_t launch_mu(int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
   switch (mu) {
-    case 0: k_phase<Real, 0, KIND, BATCHED><<<nblocks, block, 0, s>>>(lat, a); break;
-    case 1: k_phase<Real, 1, KIND, BATCHED><<<nblocks, block, 0, s>>>(lat, a); break;
-    case 2: k_phase<Real, 2, KIND, BATCHED><<<nblocks, block, 0, s>>>(lat, a); break;
-    default: k_phase<Real, 3, KIND, BATCHED><<<nblocks, block, 0, s>>>(lat, a); break;
+    case 0: k_phase<Real, 0, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;
+    case 1: k_phase<Real, 1, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;
+    case 2: k_phase<Real, 2, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;
+    default: k_phase<Real, 3, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;
   }
   return hipGetLastError();
 }
 
+template <typename Real, int KIND, bool BATCHED>
+hipError_t launch_spt(int spt, int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+  if (spt == 2) return launch_mu<Real, KIND, BATCHED, 2>(mu, lat, a, nblocks, block, s);
+  if (spt == 4) {
+    // four sites per thread only for binary32 (binary64 would exceed the register budget)
+    if (sizeof(Real) != sizeof(float)) return hipErrorInvalidValue;
+    return launch_mu<Real, KIND, BATCHED, (sizeof(Real) == sizeof(float) ? 4 : 1)>(mu, lat, a, nblocks, block, s);
+  }
+  return launch_mu<Real, KIND, BATCHED, 1>(mu, lat, a, nblocks, block, s);
+}
+
 template <typename Real>
-hipError_t launch_prec(int kind, int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+hipError_t launch_prec(int kind, int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, int spt,
+                       hipStream_t s) {
   const bool batched = a.reps != nullptr;
   if (kind == KIND_METROPOLIS)
-    return batched ? launch_mu<Real, KIND_METROPOLIS, true>(mu, lat, a, nblocks, block, s)
-                   : launch_mu<Real, KIND_METROPOLIS, false>(mu, lat, a, nblocks, block, s);
-  return batched ? launch_mu<Real, KIND_OVERRELAX, true>(mu, lat, a, nblocks, block, s)
-                 : launch_mu<Real, KIND_OVERRELAX, false>(mu, lat, a, nblocks, block, s);
+    return batched ? launch_spt<Real, KIND_METROPOLIS, true>(spt, mu, lat, a, nblocks, block, s)
+                   : launch_spt<Real, KIND_METROPOLIS, false>(spt, mu, lat, a, nblocks, block, s);
+  return batched ? launch_spt<Real, KIND_OVERRELAX, true>(spt, mu, lat, a, nblocks, block, s)
+                 : launch_spt<Real, KIND_OVERRELAX, false>(spt, mu, lat, a, nblocks, block, s);
 }
 
 }  // namespace
 
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
-                        hipStream_t s) {
-  if (prec == PREC_F64) return launch_prec<double>(kind, mu, (double*)lat_base, a, nblocks, block, s);
-  return launch_prec<float>(kind, mu, (float*)lat_base, a, nblocks, block, s);
+                        int spt, hipStream_t s) {
+  if (prec == PREC_F64) return launch_prec<double>(kind, mu, (double*)lat_base, a, nblocks, block, spt, s);
+  return launch_prec<float>(kind, mu, (float*)lat_base, a, nblocks, block, spt, s);
 }
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t s) {

@@ -20,6 +20,7 @@
 #include <string>
 
 #include "../../include/u1/array.h"
+#include "../../include/u1/checkpoint.h"
 #include "../../include/u1/multilevel.h"
 #include "../../include/u1/parameters.h"
 #include "../../include/u1/plaquette.h"
@@ -439,6 +440,26 @@ static Array<Real>* convert_dev(Array<Real>* lat, bool eo_to_no) {
 }
 Array<double>* LatticeConvert(Array<double>* lat, bool eo_to_no) { return convert_dev(lat, eo_to_no); }
 Array<float>* LatticeConvert(Array<float>* lat, bool eo_to_no) { return convert_dev(lat, eo_to_no); }
+
+// ------------------------------------------------------------------ checkpoint
+template <class Real>
+static void save_cfg(Array<Real>* lat, CudaRNG* rng, const std::string& path) {
+  const u1_geom g = CurrentGeom();
+  u1SafeCall(u1_config_save(path.c_str(), &g, prec_of<Real>(), lat->getPtr(), rng->getPtr()->seed,
+                            rng->getPtr()->sweep));
+}
+template <class Real>
+static void load_cfg(Array<Real>* lat, CudaRNG* rng, const std::string& path) {
+  const u1_geom g = CurrentGeom();
+  uint64_t seed = 0, sweep = 0;
+  u1SafeCall(u1_config_load(path.c_str(), &g, prec_of<Real>(), lat->getPtr(), &seed, &sweep));
+  rng->getPtr()->seed = seed;
+  rng->getPtr()->sweep = sweep;
+}
+void SaveConfiguration(Array<double>* lat, CudaRNG* rng, const std::string& path) { save_cfg(lat, rng, path); }
+void LoadConfiguration(Array<double>* lat, CudaRNG* rng, const std::string& path) { load_cfg(lat, rng, path); }
+void SaveConfiguration(Array<float>* lat, CudaRNG* rng, const std::string& path) { save_cfg(lat, rng, path); }
+void LoadConfiguration(Array<float>* lat, CudaRNG* rng, const std::string& path) { load_cfg(lat, rng, path); }
 
 // plaquette_comps.cu:148-227
 void PlaquetteFields(Array<double>* lat, PlaqFieldArg* pf, bool spacetime, bool evenoddOrder) {

@@ -69,7 +69,7 @@ int main() {
   // update-launch decomposition (phase_site): every site exactly once per
   // parity, right z/t, neighbour offsets equal to brute force
   long psites = 0;
-  const int blocks[] = {64, 128, 192, 256, 384, 512};
+  const int blocks[] = {64, 128, 192, 256, 384, 512, 768, 1024};  // block * sites-per-thread
   for (const auto& G : geoms) {
     u1d::RepDesc r{};
     for (int i = 0; i < 4; ++i) r.N[i] = G[i];
@@ -80,7 +80,7 @@ int main() {
     u1d::MagicDiv a = u1d::make_magic(r.nxh), b = u1d::make_magic(G[1]), c = u1d::make_magic(G[2]);
     r.div_nxh_m = a.m; r.div_nxh_s = a.s; r.div_ny_m = b.m; r.div_ny_s = b.s; r.div_nz_m = c.m; r.div_nz_s = c.s;
     for (int B : blocks) {
-      if (r.V > 1000000 && B != 192 && B != 256) continue;
+      if (r.V > 1000000 && B != 192 && B != 256 && B != 384) continue;
       u1d::set_phase_blocks(r, B);
       for (int parity = 0; parity < 2; ++parity) {
         std::vector<unsigned char> seen(r.HV, 0);

@@ -72,3 +72,24 @@ def test_tune_cache_roundtrip(tmp_path):
     stale.write_text("tunecache\tother-version\tgpu_arch=gfx950\n")
     with pytest.raises(u1.U1Error):
         u1.tune_load(str(stale))
+
+
+def test_config_file_validation_on_host(tmp_path):
+    """u1_config_load checks the header before touching the device."""
+    import struct
+
+    import latticeu1_amd as u1
+
+    g = u1.geom((4, 4, 4, 4))
+    bad = tmp_path / "bad.cfg"
+    bad.write_bytes(b"garbage" * 20)
+    with pytest.raises(u1.U1Error) as e:
+        u1.call("u1_config_load", str(bad).encode(), C.byref(g), 0, None, None, None)
+    assert e.value.code == 5
+    hdr = struct.pack("<8sI4iIddQQ", b"U1AMDCFG", 1, 4, 4, 4, 6, 0, 1.0, 1.0, 7, 3)
+    assert len(hdr) == 64
+    other = tmp_path / "other.cfg"
+    other.write_bytes(hdr)
+    with pytest.raises(u1.U1Error) as e:
+        u1.call("u1_config_load", str(other).encode(), C.byref(g), 0, None, None, None)
+    assert e.value.code == 1  # geometry mismatch

@@ -61,11 +61,20 @@ def test_c3_multihit_fp64_parity_fixed_config(u1, orc):
     c_gpu = lat.poly2(2, mh, seed=33, c1=7)
     c_orc = orc.poly2(og, ref, 2, orc.metro(1, 100, 0.5), seed=33, c1=7)
     assert np.allclose(c_gpu, c_orc, rtol=1e-11, atol=1e-18)
-    # the FP32 links on the same configuration (same Philox hits)
+    # FP32 links on the same configuration.  Analytic multihit is deterministic:
+    # its error comes from rounding the angles to binary32 and grows with the
+    # Nt = 48 factors of each loop.  The stochastic estimator is a different
+    # draw in FP32 (accept decisions on near-ties differ between precisions),
+    # so it is compared against its own statistical scale.
+    a64 = orc.poly2(og, ref, 1)
+    a32 = lat.astype(u1.F32).poly2(1)
+    rel_a = np.max(np.abs(a32 - a64)) / np.max(np.abs(a64))
     c32 = lat.astype(u1.F32).poly2(2, mh, seed=33, c1=7)
-    rel = np.max(np.abs(c32 - c_orc)) / np.max(np.abs(c_orc))
-    say(f"C3 fixed config: FP32 vs FP64 stochastic-multihit correlator, max rel diff {rel:.2e}")
-    assert rel < 1e-4
+    rel_s = np.max(np.abs(c32 - c_orc)) / np.max(np.abs(c_orc))
+    say(f"C3 fixed config FP32 vs FP64: analytic multihit correlator max rel diff {rel_a:.2e}; "
+        f"stochastic (100 hits) {rel_s:.2e}")
+    assert rel_a < 1e-5
+    assert rel_s < 1e-2
 
 
 def test_c3_run_fp32_vs_fp64(u1):

@@ -331,3 +331,27 @@ def test_tuner_records_entries(u1, tmp_path):
     u1.tune_save(str(p))
     txt = p.read_text()
     assert "Metropolis_f64" in txt and "OverRelaxation_f64" in txt
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_checkpoint_resume_is_exact(u1, tmp_path, prec):
+    """Counter-based RNG: (configuration, seed, sweep) resumes the chain exactly."""
+    dims = (8, 6, 4, 8)
+    g = u1.geom(dims, 1.0)
+    a = u1.Lattice.cold(g, prec)
+    a.hotstart(5)
+    for s in range(4):
+        a.update(1, 3, u1.BASELINE_METRO, 5, s)
+    path = str(tmp_path / "cfg.u1")
+    a.save(path, 5, 4)
+    for s in range(4, 8):
+        a.update(1, 3, u1.BASELINE_METRO, 5, s)
+    b = u1.Lattice.cold(g, prec)
+    seed, sweep = b.load(path)
+    assert (seed, sweep) == (5, 4)
+    for s in range(sweep, 8):
+        b.update(1, 3, u1.BASELINE_METRO, seed, s)
+    assert np.array_equal(a.to_host().view(np.uint8), b.to_host().view(np.uint8))
+    # the file holds FP64 angles in lexicographic order
+    raw = np.fromfile(path, dtype=np.float64, offset=64)
+    assert raw.size == 4 * int(np.prod(dims))

@@ -159,3 +159,19 @@ def test_libm_variant_agrees_short_time(orc, orc_libm):
     pa, pb = orc.plaquette(g, a), orc_libm.plaquette(g, b)
     assert np.max(np.abs(pa - pb)) < 1e-10
     assert np.max(np.abs(np.angle(np.exp(1j * (a - b))))) < 1e-7
+
+
+def test_gamma_method_on_ar1(tmp_path):
+    """Drop-in actime.h: tau_int of AR(1) series is (1 + rho) / (2 (1 - rho))."""
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "cpu_checks", "check_gamma.cpp")
+    exe = tmp_path / "check_gamma"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include", "u1"), src, "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in filter(None, out):
+        rho, exact, tau, dtau, dval, dexact, w, ok = (float(v) for v in line.split())
+        assert ok == 1
+        assert abs(tau - exact) < 3 * dtau + 0.02, line
+        assert abs(dval / dexact - 1) < 0.1, line
