@@ -48,6 +48,37 @@ def jackknife(x, nbins=20):
     return float(x[:n].mean()), err
 
 
+def gamma_method(a, S=1.5):
+    """Wolff's Gamma method (hep-lat/0306017): mean, error with autocorrelations,
+    integrated autocorrelation time, window.  Same algorithm as include/u1/actime.h."""
+    a = np.asarray(a, dtype=np.float64)
+    N = len(a)
+    d = a - a.mean()
+    tmax = N // 2
+    G = [float(np.dot(d, d) / N)]
+    sum_rho, W, found = 0.0, 0, False
+    t = 1
+    while t <= tmax:
+        G.append(float(np.dot(d[:N - t], d[t:]) / (N - t)))
+        if not found:
+            if G[0] <= 0:
+                break
+            sum_rho += G[t] / G[0]
+            tau = 1e-300 if sum_rho <= 0 else S / math.log((sum_rho + 1) / sum_rho)
+            if math.exp(-t / tau) - tau / math.sqrt(t * N) < 0:
+                W, found = t, True
+                tmax = min(tmax, 2 * t)
+        t += 1
+    if not found:
+        W = len(G) - 1
+    if G[0] <= 0:
+        return float(a.mean()), 0.0, 0.5, W
+    C = G[0] + 2 * sum(G[1:W + 1])
+    G = [g + C / N for g in G]
+    C = G[0] + 2 * sum(G[1:W + 1])
+    return float(a.mean()), math.sqrt(C / N), C / (2 * G[0]), W
+
+
 def run_path(u1, prec, dims, beta, seed, warmup, steps):
     g = u1.geom(dims, beta, 1.0)
     lat = u1.Lattice.cold(g, prec)
@@ -117,10 +148,12 @@ def main():
         lat, ms, kms, series = run_path(u1, prec, dims, args.beta, args.seed, args.warmup, args.steps)
         pm = np.array([u1.mean_plaquette(r) for r in series])
         mean, err = jackknife(pm)
+        _, gerr, tau, _ = gamma_method(pm)
+        err = max(err, gerr)  # binned jackknife, or the Gamma-method error if larger
         s = 4 if prec == u1.F32 else 8
         bytes_per_launch = 4.5 * V * s  # 4V angles read + V/2 written per (parity, mu) phase
         res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, plaq=mean,
-                         plaq_err=err, bytes=bytes_per_launch, elem=s)
+                         plaq_err=err, tau=tau, bytes=bytes_per_launch, elem=s)
         print(f"[bench] {name}: {res[name]['rate']:.4e} link updates/s, {ms / args.steps:.3f} ms/sweep, "
               f"<P> = {mean:.8f} +- {err:.1e}, mean phase kernel ms M/OR = {kms[0]:.4f}/{kms[1]:.4f}",
               file=sys.stderr, flush=True)
@@ -147,6 +180,7 @@ def main():
         d = f32["plaq"] - f64["plaq"]
         comb = math.hypot(f32["plaq_err"], f64["plaq_err"])
         extra["plaquette"] = {"fp32": [f32["plaq"], f32["plaq_err"]], "fp64": [f64["plaq"], f64["plaq_err"]],
+                              "tau_int_sweeps": {"fp32": f32["tau"], "fp64": f64["tau"]},
                               "diff": d, "diff_over_combined_err": d / comb if comb > 0 else None,
                               "within_1sigma_of_fp64": abs(d) <= f64["plaq_err"]}
         # fixed configuration: FP32 measurement of the final FP64 configuration

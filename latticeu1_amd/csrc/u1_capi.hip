@@ -22,6 +22,12 @@ namespace {
 
 thread_local std::string g_err;
 
+// Launch shape of an update phase kernel: workgroup size, even/odd sites per
+// thread, XCD-aware workgroup order.
+struct Cfg {
+  int block, spt, xcd;
+};
+
 struct Ctx {
   bool init = false;
   int device = 0;
@@ -31,7 +37,7 @@ struct Ctx {
   unsigned long long* counter = nullptr;
   double* dres = nullptr;  // small device result buffer (64 doubles)
   bool tune = true;
-  std::map<std::string, std::pair<int, int>> tunecache;  // key -> (block, sites per thread)
+  std::map<std::string, Cfg> tunecache;  // key -> launch shape
   int tune_loaded = 0;
 };
 Ctx G;
@@ -133,12 +139,20 @@ int ensure_partials(size_t n) {
 // of the lattice, so tuning never perturbs the chain (the reference backs up
 // and restores lattice and RNG instead, tune.h preTune/postTune).  Results do
 // not depend on the launch shape.
-struct Cfg {
-  int block, spt;
-};
-const Cfg kCandidates[] = {{64, 1}, {128, 1}, {192, 1}, {256, 1}, {384, 1}, {512, 1}, {64, 2},
-                           {128, 2}, {192, 2}, {256, 2}, {384, 2}, {64, 4},  {128, 4}, {192, 4}, {256, 4}};
-const Cfg kDefaultCfg = {256, 1};
+const Cfg kShapes[] = {{64, 1, 0},  {128, 1, 0}, {192, 1, 0}, {256, 1, 0}, {384, 1, 0},
+                       {512, 1, 0}, {64, 2, 0},  {128, 2, 0}, {192, 2, 0}, {256, 2, 0},
+                       {384, 2, 0}, {64, 4, 0},  {128, 4, 0}, {192, 4, 0}, {256, 4, 0}};
+// every shape is tried with and without the XCD-aware workgroup order
+std::vector<Cfg> candidates() {
+  std::vector<Cfg> v;
+  for (int x = 0; x < 2; ++x)
+    for (Cfg c : kShapes) {
+      c.xcd = x;
+      v.push_back(c);
+    }
+  return v;
+}
+const Cfg kDefaultCfg = {256, 1, 1};
 
 std::string vol_str(const u1_geom* g) {
   std::ostringstream s;
@@ -171,7 +185,7 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
   const std::string key = tune_key(name, g, prec);
   auto it = G.tunecache.find(key);
   if (it != G.tunecache.end()) {
-    *cfg = {it->second.first, it->second.second};
+    *cfg = it->second;
     return U1_OK;
   }
   if (!G.tune) {
@@ -184,10 +198,11 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
   HIPCHK(hipMemcpyAsync(scratch, lat, bytes, hipMemcpyDeviceToDevice, G.st));
   float best = 1e30f;
   Cfg bestc = kDefaultCfg;
-  for (const Cfg c : kCandidates) {
+  for (const Cfg c : candidates()) {
     if (c.spt == 4 && prec != U1_F32) continue;
     PhaseArgs a = proto;
     a.accepts = nullptr;
+    a.xcd = (uint32_t)c.xcd;
     set_blocks(a.single, c.block * c.spt);
     float ms = 0;
     int rc = time_phase(prec, kind, scratch, a, a.single.nblk, c, &ms);
@@ -201,7 +216,7 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
     }
   }
   HIPCHK(hipFree(scratch));
-  G.tunecache[key] = {bestc.block, bestc.spt};
+  G.tunecache[key] = bestc;
   *cfg = bestc;
   return U1_OK;
 }
@@ -236,6 +251,7 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   set_blocks(ro, co.block * co.spt);
   if (gev) HIPCHK(hipEventRecord(gev->start, G.st));
   a.single = rm;
+  a.xcd = (uint32_t)cm.xcd;
   for (int p = 0; p < metrop; ++p)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
@@ -245,6 +261,7 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
       }
   if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
   a.single = ro;
+  a.xcd = (uint32_t)co.xcd;
   for (int o = 0; o < ovrn; ++o)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
@@ -295,7 +312,7 @@ struct u1_ensemble {
   double* partials = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  Cfg cfg = {256, 1};  // launch shape of the update phases (tuned on first use)
+  Cfg cfg = {256, 1, 1};  // launch shape of the update phases (tuned on first use)
   bool tuned = false;
 };
 
@@ -338,6 +355,7 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
   a.sweep = 0;
   a.sweep_dev = e->sweep_dev;
   a.accepts = nullptr;
+  a.xcd = (uint32_t)e->cfg.xcd;
   for (int p = 0; p < e->metrop; ++p)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
@@ -375,7 +393,7 @@ int ensemble_tune(u1_ensemble* e) {
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   Cfg bestc = e->cfg;
-  for (const Cfg c : kCandidates) {
+  for (const Cfg c : candidates()) {
     if (c.spt == 4 && e->prec != U1_F32) continue;
     RET(ensemble_build_update(e, c));
     RET(ensemble_enqueue_phases(e, scratch));  // warm-up
@@ -563,6 +581,7 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
   Cfg c = kDefaultCfg;
   RET(phase_cfg(prec, kind, g, lat, a, &c));
   set_blocks(a.single, c.block * c.spt);
+  a.xcd = (uint32_t)c.xcd;
   HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, c.block, c.spt, G.st));
   return U1_OK;
 }
@@ -596,8 +615,18 @@ int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, co
   if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad Polyakov mode");
   if (mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
   const RepDesc r = make_rep(g, seed, c3);
-  HIPCHK(u1d::launch_poly_volume(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0, mh ? mh->step : 0.0,
-                                 stream, c1, c3, (double*)P_dev, 256, G.st));
+  if (mode == U1_POLY_PLAIN) {
+    HIPCHK(u1d::launch_poly_volume(prec, lat, r, mode, 0, 0, 0.0, stream, c1, c3, (double*)P_dev, 256, G.st));
+    return U1_OK;
+  }
+  // multihit: one thread per temporal link (V-wide), then the product along t
+  double* tl = nullptr;
+  HIPCHK(hipMalloc((void**)&tl, 2 * (size_t)r.V * sizeof(double)));
+  HIPCHK(u1d::launch_temporal_links(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0,
+                                    mh ? mh->step : 0.0, stream, c1, c3, tl, 256, G.st));
+  HIPCHK(u1d::launch_tprod(tl, r, (double*)P_dev, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipFree(tl));
   return U1_OK;
 }
 
@@ -1007,7 +1036,7 @@ int u1_tune_load(const char* path) {
     if (!(std::getline(ls, vol, '\t') && std::getline(ls, name, '\t') && std::getline(ls, aux, '\t')))
       return fail(U1_ERR_IO, std::string("Bad format in ") + path);
     if (!(ls >> bx >> by >> bz >> gx >> gy >> gz >> sh)) return fail(U1_ERR_IO, std::string("Bad format in ") + path);
-    G.tunecache[vol + "\t" + name] = {bx, (by == 2 || by == 4) ? by : 1};
+    G.tunecache[vol + "\t" + name] = Cfg{bx, (by == 2 || by == 4) ? by : 1, bz == 2 ? 1 : 0};
     ++n;
   }
   G.tune_loaded = n;
@@ -1024,8 +1053,9 @@ int u1_tune_save(const char* path) {
     const std::string& key = kv.first;
     const size_t tab = key.find('\t');
     const std::string vol = key.substr(0, tab), name = key.substr(tab + 1);
-    f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.first << "\t" << kv.second.second
-      << "\t1\t0\t1\t1\t0\t# block.x = workgroup size, block.y = even/odd sites per thread\n";
+    f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.block << "\t" << kv.second.spt << "\t"
+      << (kv.second.xcd ? 2 : 1)
+      << "\t0\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z = 2: XCD-aware order\n";
   }
   return f ? U1_OK : fail(U1_ERR_IO, "write failed");
 }

@@ -126,6 +126,17 @@ __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b
   s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
 }
 
+// XCD-aware workgroup order.  The dispatcher hands workgroup b of a launch to
+// XCD b % 8 (round robin), and each XCD has its own L2.  Returning logical
+// index L such that XCD x gets the contiguous range
+// [x*q + min(x, rem), ...) of logical indices (q = n / 8, rem = n % 8) keeps
+// neighbouring (z, t) planes -- which share halo data -- in one L2.  It is a
+// bijection of [0, n).
+__host__ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+  const uint32_t x = b & 7u, q = n >> 3, rem = n & 7u;
+  return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
 // Host helper: workgroup layout of update launches for a block size.
 inline void set_phase_blocks(RepDesc& r, int block) {
   const int P = r.nxh * r.N[1];

@@ -20,7 +20,7 @@ struct PhaseArgs {
   int32_t nhit;
   double step;
   uint32_t stream, pass, sweep;
-  uint32_t pad;
+  uint32_t xcd;                 // 1: XCD-aware workgroup order (xcd_remap)
   const uint32_t* sweep_dev;    // if set, the sweep index is read here (graph replays)
   unsigned long long* accepts;  // if set, accepted hits are added here
 };
@@ -48,6 +48,8 @@ hipError_t launch_poly_volume(int prec, const void* lat, const RepDesc& r, int m
 hipError_t launch_temporal_links(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
                                  double step, uint32_t stream, uint32_t c1, uint32_t c3, double* out, int block,
                                  hipStream_t st);
+// P(x) = prod_t T(x, t) from a temporal-link field (2*V doubles, lexicographic 4D)
+hipError_t launch_tprod(const double* tl, const RepDesc& r, double* P, hipStream_t st);
 hipError_t launch_pp(const double* P, const RepDesc& r, int rmax, double* partials /*2*rmax*nblk*/, int nblk,
                      int block, hipStream_t st);
 hipError_t launch_convert_layout(int prec, const void* in, void* out, const RepDesc& r, int eo_to_no, int block,

@@ -225,6 +225,20 @@ __global__ __launch_bounds__(256) void k_temporal_links(const Real* __restrict__
   out[2 * j + 1] = v.im;
 }
 
+// Product over t of a per-site temporal-link field (lexicographic 4D index),
+// in t order as polyakov.cu:430-437: P(x) = prod_t T(x, t).
+__global__ __launch_bounds__(256) void k_tprod(const double* __restrict__ tl, RepDesc r, double* __restrict__ P) {
+  const int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= r.SV) return;
+  C64 res = {1.0, 0.0};
+  for (int t = 0; t < r.N[3]; ++t) {
+    const int64_t k = j + (int64_t)t * r.SV;
+    res = cmul(res, C64{tl[2 * k], tl[2 * k + 1]});
+  }
+  P[2 * (int64_t)j] = res.re;
+  P[2 * (int64_t)j + 1] = res.im;
+}
+
 // polyakov.cu:514-535: sum over x and the three spatial axes of
 // P(x) conj(P(x + R e_dir)), R = 1..rmax; one partial row of 2*rmax per workgroup.
 __global__ __launch_bounds__(256) void k_pp(const double* __restrict__ P, RepDesc r, int rmax,
@@ -470,6 +484,11 @@ hipError_t launch_poly_volume(int prec, const void* lat, const RepDesc& r, int m
     k_poly_volume<double><<<nb, block, 0, st>>>((const double*)lat, r, mode, proposal, nhit, step, stream, c1, c3, P);
   else
     k_poly_volume<float><<<nb, block, 0, st>>>((const float*)lat, r, mode, proposal, nhit, step, stream, c1, c3, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_tprod(const double* tl, const RepDesc& r, double* P, hipStream_t st) {
+  k_tprod<<<nblocks(r.SV, 256), 256, 0, st>>>(tl, r, P);
   return hipGetLastError();
 }
 

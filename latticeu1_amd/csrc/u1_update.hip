@@ -121,8 +121,9 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // past the end of a plane are pointed at site 0 (in bounds) and do not store.
 template <typename Real, int MU, int KIND, bool BATCHED, int SPT>
 __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, PhaseArgs a) {
-  const RepDesc& r = BATCHED ? a.reps[a.blockmap[blockIdx.x]] : a.single;
-  const uint32_t b = blockIdx.x - (uint32_t)r.blk_begin;
+  const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
+  const uint32_t b = L - (uint32_t)r.blk_begin;
   PhaseSite ps[SPT];
 #pragma unroll
   for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, a.parity, ps[k]);

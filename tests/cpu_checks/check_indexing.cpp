@@ -66,6 +66,15 @@ int main() {
           }
       }
   }
+  // XCD-aware workgroup order is a bijection of [0, n)
+  for (uint32_t n = 1; n < 3000; n += (n < 64 ? 1 : 37)) {
+    std::vector<unsigned char> hit(n, 0);
+    for (uint32_t b = 0; b < n; ++b) {
+      const uint32_t L = u1d::xcd_remap(b, n);
+      if (L >= n || hit[L]) { printf("xcd_remap not a bijection n=%u b=%u\n", n, b); return 1; }
+      hit[L] = 1;
+    }
+  }
   // update-launch decomposition (phase_site): every site exactly once per
   // parity, right z/t, neighbour offsets equal to brute force
   long psites = 0;
