@@ -5,7 +5,9 @@ Workload (one "step" = one sweep of the hot path over the whole lattice):
   48^4 compact U(1), Wilson action, beta = 1.0, hot start;
   sweep = 1 Metropolis pass (4 hits, theta' = theta + U[-0.5, 0.5)) + 3
   overrelaxation passes, each pass = 8 checkerboard (parity, mu) launches;
-  the plaquette is measured after every sweep (FP64 reduction, on the device).
+  the plaquette is measured after every sweep (FP64 reduction, on the device),
+  by default summed inside the last overrelaxation pass (--plaq fused) from
+  the staple pieces and the new angles, or by a separate pass (--plaq separate).
 Defaults follow the 48^4 protocol: 500 thermalisation sweeps (--warmup) then
 1000 measured, timed sweeps (--steps), for the FP32-bulk path and again for
 the FP64 path on the same GPU.  A link update is one link visited by one pass,
@@ -79,14 +81,14 @@ def gamma_method(a, S=1.5):
     return float(a.mean()), math.sqrt(C / N), C / (2 * G[0]), W
 
 
-def run_path(u1, prec, dims, beta, seed, warmup, steps):
+def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
     g = u1.geom(dims, beta, 1.0)
     lat = u1.Lattice.cold(g, prec)
     lat.hotstart(seed)
     m = u1.BASELINE_METRO
     if warmup > 0:
-        lat.bench(1, 3, m, seed, 0, warmup, False)  # thermalisation (also runs the autotuner)
-    ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, True)
+        lat.bench(1, 3, m, seed, 0, warmup, 0)  # thermalisation (also runs the autotuner)
+    ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, plaq_mode)
     return lat, ms, kernel_ms, series
 
 
@@ -122,6 +124,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--plaq", choices=["fused", "separate"], default="fused",
+                    help="plaquette every sweep: summed inside the last overrelaxation pass (fused) or by a "
+                         "separate pass over the lattice")
+    ap.add_argument("--tunecache", type=str, default="",
+                    help="launch-shape cache: loaded if the file exists (no tuning launches), else written after "
+                         "tuning")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the run, copy a buffer of this many MiB once (plain copy kernel of known size, "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -138,6 +146,8 @@ def main():
 
     u1.init(0)
     dev = u1.device_name()
+    if args.tunecache and os.path.exists(args.tunecache):
+        u1.tune_load(args.tunecache)
 
     res = {}
     for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
@@ -145,7 +155,8 @@ def main():
             continue
         print(f"[bench] {name}: {args.warmup} thermalisation + {args.steps} timed sweeps at {dims}", file=sys.stderr,
               flush=True)
-        lat, ms, kms, series = run_path(u1, prec, dims, args.beta, args.seed, args.warmup, args.steps)
+        lat, ms, kms, series = run_path(u1, prec, dims, args.beta, args.seed, args.warmup, args.steps,
+                                        1 if args.plaq == "fused" else 2)
         pm = np.array([u1.mean_plaquette(r) for r in series])
         mean, err = jackknife(pm)
         _, gerr, tau, _ = gamma_method(pm)
@@ -189,6 +200,9 @@ def main():
         extra["fixed_config_plaquette_rel_err"] = float(abs(u1.mean_plaquette(p32) - u1.mean_plaquette(p64))
                                                         / abs(u1.mean_plaquette(p64)))
 
+    if args.tunecache and not os.path.exists(args.tunecache):
+        u1.tune_save(args.tunecache)
+
     if args.calib_mb > 0:
         n = int(args.calib_mb * 1048576 / 4)
         src, dst = u1.DeviceArray(n, np.float32), u1.DeviceArray(n, np.float32)
@@ -217,6 +231,8 @@ def main():
         "config": {"workload": "U(1) Wilson gauge, 48^4, beta=1.0, hot start; sweep = 1 Metropolis "
                                "(4 hits, delta U[-0.5,0.5)) + 3 overrelaxation; plaquette every sweep",
                    "lattice": "x".join(str(d) for d in dims), "precision": "fp32 angles, fp64 reductions",
+                   "plaquette": ("summed inside the last overrelaxation pass" if args.plaq == "fused"
+                                 else "separate pass"),
                    "link_updates_per_sweep": lu_per_sweep, "device": dev},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -19,6 +19,7 @@
  *   u1_malloc/u1_free/u1_memcpy  dev_malloc, Array::Copy     include/alloc.h:60-67, include/array.h:60-82
  *   u1_hotstart                  HotStart                    include/update.h:10
  *   u1_update                    UpdateLattice               include/update.h:12
+ *   u1_update_plaquette          UpdateLattice + Plaquette   include/update.h:12, include/plaquette.h:12
  *   u1_plaquette                 Plaquette                   include/plaquette.h:12
  *   u1_polyakov                  Polyakov                    include/polyakov.h:13
  *   u1_poly_volume, u1_pp,       Poly2 (Polyakov_Vol + PP)   include/polyakov.h:16, src/polyakov.cu:618-654
@@ -105,6 +106,17 @@ int u1_hotstart(const u1_geom* g, int prec, void* lat, uint64_t seed, uint32_t r
 int u1_update(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
               uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t c3, int32_t mask_ml, uint64_t* accepts);
 
+/* u1_update (mask_ml 0, no accept count) fused with the plaquette of the
+ * updated lattice, returned in u1_plaquette's form.  The plaquettes are
+ * summed inside the last pass's (parity 1, mu = 1, 2, 3) launches from the
+ * staple pieces and the new angles, saving a separate pass over the lattice.
+ * Counterpart of the reference's UpdateLattice followed by Plaquette
+ * (include/update.h:12, include/plaquette.h:12).  Binary64: equal to
+ * u1_plaquette up to summation order; binary32: within ~1e-6 relative (the
+ * pieces use the update kernels' hardware sin/cos). */
+int u1_update_plaquette(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+                        uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t c3, double out[4]);
+
 /* One checkerboard phase: a single launch of kernel_metropolis (kind 0, with
  * Metropolis pass index `pass`) or kernel_overrelaxation (kind 1) on parity
  * `parity`, direction mu (update.cu:177-190, 227-231). */
@@ -185,7 +197,8 @@ int u1_tune_entries(int32_t* n);
 
 /* ---------------------------------------------------------------- timing helper */
 /* Run nsweeps of u1_update (sweep indices sweep0, sweep0+1, ...; a plaquette
- * measurement into plaq_series[4*i] after every sweep if measure_plaq) and
+ * measurement into plaq_series[4*i] after every sweep if measure_plaq: 1 fused
+ * into the sweep as u1_update_plaquette, 2 a separate u1_plaquette pass) and
  * return the elapsed device time in ms, from HIP events on the library stream.
  * kernel_ms (nullable): mean launch duration of the Metropolis (kernel_ms[0])
  * and overrelaxation (kernel_ms[1]) phase kernels, from events that bracket

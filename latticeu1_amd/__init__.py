@@ -76,6 +76,8 @@ _SIGS = {
     "u1_hotstart": (C.POINTER(Geom), C.c_int, _P, C.c_uint64, C.c_uint32),
     "u1_update": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.c_int32, C.POINTER(Metro), C.c_uint64, C.c_uint32,
                   C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)),
+    "u1_update_plaquette": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.c_int32, C.POINTER(Metro), C.c_uint64,
+                            C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)),
     "u1_phase": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Metro), C.c_uint64,
                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32),
     "u1_selftest_math": (C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64),
@@ -256,6 +258,14 @@ class Lattice:
              mask_ml, C.byref(acc) if count_accepts else None)
         return int(acc.value)
 
+    def update_plaquette(self, metrop: int, ovrn: int, m: Metro, seed: int, sweep: int, stream: int = STREAM_UPDATE,
+                         c3: int = 0) -> np.ndarray:
+        """update() fused with the plaquette of the result (u1_plaquette form)."""
+        out = np.zeros(4)
+        call("u1_update_plaquette", C.byref(self.g), self.prec, self.buf.ptr, metrop, ovrn, C.byref(m), seed, stream,
+             sweep, c3, _dp(out))
+        return out
+
     def phase(self, kind: int, parity: int, mu: int, m: Metro, seed: int, sweep: int = 0, pass_: int = 0,
               stream: int = STREAM_UPDATE, c3: int = 0, mask_ml: int = 0) -> None:
         call("u1_phase", C.byref(self.g), self.prec, self.buf.ptr, kind, parity, mu, C.byref(m), seed, stream, sweep,
@@ -338,12 +348,14 @@ class Lattice:
         h = F.to_host()
         return (h[0::2] + 1j * h[1::2]).reshape(6, size), means[0::2] + 1j * means[1::2]
 
-    def bench(self, metrop: int, ovrn: int, m: Metro, seed: int, sweep0: int, nsweeps: int, measure_plaq: bool):
+    def bench(self, metrop: int, ovrn: int, m: Metro, seed: int, sweep0: int, nsweeps: int, measure_plaq: int):
+        """measure_plaq: 0 none, 1 fused into the sweep, 2 separate plaquette pass."""
+        measure_plaq = int(measure_plaq)
         series = np.zeros(4 * nsweeps) if measure_plaq else None
         el = C.c_double(0)
         km = np.zeros(2)
         call("u1_bench_updates", C.byref(self.g), self.prec, self.buf.ptr, metrop, ovrn, C.byref(m), seed, sweep0,
-             nsweeps, 1 if measure_plaq else 0, _dp(series) if measure_plaq else None, C.byref(el), _dp(km))
+             nsweeps, measure_plaq, _dp(series) if measure_plaq else None, C.byref(el), _dp(km))
         return el.value, km, (series.reshape(nsweeps, 4) if measure_plaq else None)
 
 

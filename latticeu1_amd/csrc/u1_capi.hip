@@ -132,6 +132,33 @@ int ensure_partials(size_t n) {
   return U1_OK;
 }
 
+// Room for nblk rows of nv partial sums followed by the fold's scratch rows.
+int ensure_fold(int nblk, int nv) { return ensure_partials((size_t)nv * nblk + u1d::fold_scratch(nblk, nv)); }
+double* fold_scratch_ptr(int nblk, int nv) { return G.partials + (size_t)nv * nblk; }
+
+// Launch shape of the plaquette: the (workgroup, sites per thread) pair whose
+// chunks tile a (z, t) plane of P = Nx/2 * Ny sites with the fewest idle
+// slots, then the most sites per workgroup (fewer partial rows).
+void plaquette_shape(const RepDesc& r, int prec, int* block, int* spt) {
+  const int P = r.nxh * r.N[1];
+  long best_slots = -1;
+  int bb = 64, bs = 1;
+  for (int s : {1, 2, 4}) {
+    if (s == 4 && prec == U1_F64) continue;
+    for (int b : {64, 128, 192, 256}) {
+      const long chunk = (long)b * s;
+      const long slots = (P + chunk - 1) / chunk * chunk;
+      if (best_slots < 0 || slots < best_slots || (slots == best_slots && chunk > (long)bb * bs)) {
+        best_slots = slots;
+        bb = b;
+        bs = s;
+      }
+    }
+  }
+  *block = bb;
+  *spt = bs;
+}
+
 // ------------------------------------------------------------------ autotuner
 // Own tuner (replacing the QUDA-derived tune.cpp:306-416): launch shape per
 // (kernel class, geometry, precision) -- workgroup size and even/odd sites per
@@ -228,9 +255,12 @@ struct GroupEvents {
 };
 
 // One UpdateLattice on a single lattice (update.cu:449-456 loop order).
+// meas_dst (device, 4 doubles, nullable): the plaquette of the updated
+// lattice, in u1_plaquette's normalisation, from the measuring variants of
+// phases (parity 1, mu = 1, 2, 3) of the sweep's last pass.
 int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, const u1_metro* m, uint64_t seed,
                uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
-               const GroupEvents* gev = nullptr) {
+               const GroupEvents* gev = nullptr, double* meas_dst = nullptr) {
   PhaseArgs a{};
   a.single = make_rep(g, seed, c3);
   a.reps = nullptr;
@@ -249,6 +279,14 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   RepDesc rm = a.single, ro = a.single;
   set_blocks(rm, cm.block * cm.spt);
   set_blocks(ro, co.block * co.spt);
+  if (meas_dst && (ml != 0 || metrop + ovrn == 0)) return fail(U1_ERR_ARG, "no measuring pass");
+  const int meas_nblk = ovrn > 0 ? ro.nblk : rm.nblk;  // rows per measuring phase
+  if (meas_dst) RET(ensure_fold(3 * meas_nblk, 4));
+  // measuring phases: parity 1, mu >= 1 of the last pass
+  auto meas_rows = [&](bool last_pass, int parity, int mu) -> double* {
+    if (!meas_dst || !last_pass || parity != 1 || mu < 1) return nullptr;
+    return G.partials + 4 * (size_t)meas_nblk * (size_t)(mu - 1);
+  };
   if (gev) HIPCHK(hipEventRecord(gev->start, G.st));
   a.single = rm;
   a.xcd = (uint32_t)cm.xcd;
@@ -257,6 +295,7 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)p;
         a.parity = parity;
+        a.meas = meas_rows(ovrn == 0 && p == metrop - 1, parity, mu);
         HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, rm.nblk, cm.block, cm.spt, G.st));
       }
   if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
@@ -267,20 +306,28 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
+        a.meas = meas_rows(o == ovrn - 1, parity, mu);
         HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, ro.nblk, co.block, co.spt, G.st));
       }
   if (gev) HIPCHK(hipEventRecord(gev->after_or, G.st));
+  if (meas_dst)
+    HIPCHK(u1d::launch_finish_sums(G.partials, 3 * meas_nblk, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
+                                   fold_scratch_ptr(3 * meas_nblk, 4), G.st));
   return U1_OK;
 }
 
 // Plaquette into a device result slot dst[4] (no synchronisation).
 int plaquette_async(const u1_geom* g, int prec, const void* lat, double* dst) {
-  const RepDesc r = make_rep(g, 0, 0);
-  const int block = 256;
-  const int nblk = (r.V + block - 1) / block;
-  RET(ensure_partials(4 * (size_t)nblk));
-  HIPCHK(u1d::launch_plaquette(prec, lat, r, G.partials, nblk, block, G.st));
-  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 4, 1.0 / (3.0 * (double)r.V), dst, G.st));
+  RepDesc r = make_rep(g, 0, 0);
+  int block, spt;
+  plaquette_shape(r, prec, &block, &spt);
+  set_blocks(r, block * spt);
+  r.nblk *= 2;  // even sites, then odd sites
+  const int nblk = r.nblk;
+  RET(ensure_fold(nblk, 4));
+  HIPCHK(u1d::launch_plaquette(prec, lat, r, G.partials, nblk, block, spt, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 4, 1.0 / (3.0 * (double)r.V), dst, fold_scratch_ptr(nblk, 4),
+                                 G.st));
   return U1_OK;
 }
 
@@ -318,7 +365,8 @@ struct u1_ensemble {
 
 namespace {
 
-const int kEnsBlock = 256;  // measurement launches
+const int kEnsBlock = 64;  // measurement launches: workgroup size and sites per thread
+const int kEnsSpt = 1;     // (small plane chunks suit the small replicas of a batch)
 
 // (Re)build the update-launch tables for launch shape c: per-replica plane
 // decomposition and first workgroup, and the workgroup -> replica map.
@@ -558,6 +606,18 @@ int u1_update(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovr
   return U1_OK;
 }
 
+int u1_update_plaquette(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
+                        uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t c3, double out[4]) {
+  REQ_INIT();
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  RET(check_metro(m));
+  if (!lat || !out) return fail(U1_ERR_ARG, "null argument");
+  if (metrop < 0 || ovrn < 0 || metrop > 4095 || metrop + ovrn == 0) return fail(U1_ERR_ARG, "bad pass counts");
+  RET(run_update(g, prec, lat, metrop, ovrn, m, seed, stream, sweep, c3, 0, nullptr, nullptr, G.dres));
+  return copy_back(out, G.dres, 4);
+}
+
 int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity, int32_t mu, const u1_metro* m,
              uint64_t seed, uint32_t stream, uint32_t sweep, uint32_t pass, uint32_t c3, int32_t mask_ml) {
   REQ_INIT();
@@ -601,9 +661,9 @@ int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]) {
   RET(check_prec(prec));
   const RepDesc r = make_rep(g, 0, 0);
   const int block = 256, nblk = (r.SV + block - 1) / block;
-  RET(ensure_partials(2 * (size_t)nblk));
+  RET(ensure_fold(nblk, 2));
   HIPCHK(u1d::launch_polyakov(prec, lat, r, G.partials, nblk, block, G.st));
-  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2, 1.0 / (double)r.SV, G.dres, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2, 1.0 / (double)r.SV, G.dres, fold_scratch_ptr(nblk, 2), G.st));
   return copy_back(out, G.dres, 2);
 }
 
@@ -649,9 +709,10 @@ int u1_pp(const u1_geom* g, const void* P_dev, int32_t rmax, double* out) {
   if (rmax < 1 || rmax > 32) return fail(U1_ERR_ARG, "rmax must be in [1, 32]");
   const RepDesc r = make_rep(g, 0, 0);
   const int block = 256, nblk = (r.SV + block - 1) / block;
-  RET(ensure_partials(2 * (size_t)rmax * nblk));
+  RET(ensure_fold(nblk, 2 * rmax));
   HIPCHK(u1d::launch_pp((const double*)P_dev, r, rmax, G.partials, nblk, block, G.st));
-  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres,
+                                 fold_scratch_ptr(nblk, 2 * rmax), G.st));
   return copy_back(out, G.dres, 2 * (size_t)rmax);
 }
 
@@ -734,11 +795,12 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   }
   if (rc == U1_OK) {
     const int block = 256, nblk = (r.SV + block - 1) / block;
-    rc = ensure_partials(2 * (size_t)rmax * nblk);
+    rc = ensure_fold(nblk, 2 * rmax);
     if (rc == U1_OK) {
       MLCHK(u1d::launch_ml_final((const double*)l4, (double*)ppspace_dev, r, rmax, l4norm, G.partials, nblk, block,
                                  G.st));
-      MLCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres, G.st));
+      MLCHK(u1d::launch_finish_sums(G.partials, nblk, 2 * rmax, 1.0 / (3.0 * (double)r.SV), G.dres,
+                                    fold_scratch_ptr(nblk, 2 * rmax), G.st));
       rc = copy_back(out, G.dres, 2 * (size_t)rmax);
     }
   }
@@ -765,10 +827,11 @@ int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spa
   const RepDesc r = make_rep(g, 0, 0);
   const int size = spacetime ? r.V : r.SV;
   const int block = 256, nblk = (size + block - 1) / block;
-  RET(ensure_partials(12 * (size_t)nblk));
+  RET(ensure_fold(nblk, 12));
   HIPCHK(u1d::launch_plaquette_fields(prec, lat, r, spacetime, eo_order, (double*)field_dev, G.partials, nblk, block,
                                       G.st));
-  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 12, 1.0 / (double)size, G.dres, G.st));
+  HIPCHK(u1d::launch_finish_sums(G.partials, nblk, 12, 1.0 / (double)size, G.dres, fold_scratch_ptr(nblk, 12),
+                                 G.st));
   return copy_back(means, G.dres, 12);
 }
 
@@ -877,8 +940,9 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
     r.lat_off = (uint64_t)off;
     off += 4 * (int64_t)r.V;
     RepDesc u = r, q = r;
+    set_blocks(q, kEnsBlock * kEnsSpt);
+    q.nblk *= 2;  // even sites, then odd sites
     q.blk_begin = e->meas_blocks;
-    q.nblk = (r.V + kEnsBlock - 1) / kEnsBlock;
     e->meas_blocks += q.nblk;
     for (int b = 0; b < q.nblk; ++b) mmap.push_back((uint16_t)i);
     e->upd_host.push_back(u);
@@ -947,7 +1011,7 @@ int u1_ensemble_plaquette(u1_ensemble* e, double* out) {
   REQ_INIT();
   if (!e || !out) return fail(U1_ERR_ARG, "null argument");
   HIPCHK(u1d::launch_plaquette_batched(e->prec, e->lat, e->meas_dev, e->meas_map, e->partials, e->meas_blocks,
-                                       kEnsBlock, G.st));
+                                       kEnsBlock, kEnsSpt, G.st));
   HIPCHK(u1d::launch_batched_finish(e->partials, e->meas_dev, e->nrep, e->res_dev, G.st));
   return copy_back(out, e->res_dev, 4 * (size_t)e->nrep);
 }
@@ -1075,6 +1139,8 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   RET(check_prec(prec));
   RET(check_metro(m));
   if (nsweeps < 1) return fail(U1_ERR_ARG, "nsweeps must be >= 1");
+  if (measure_plaq < 0 || measure_plaq > 2) return fail(U1_ERR_ARG, "measure_plaq must be 0, 1 or 2");
+  if (measure_plaq == 1 && metrop + ovrn == 0) return fail(U1_ERR_ARG, "no measuring pass");
   double* series = nullptr;
   if (measure_plaq) HIPCHK(hipMalloc((void**)&series, 4 * (size_t)nsweeps * sizeof(double)));
   {  // tune outside the timed region
@@ -1102,8 +1168,8 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   HIPCHK(hipEventRecord(e0, G.st));
   for (int s = 0; s < nsweeps; ++s) {
     RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)s, 0, 0, nullptr,
-                   kernel_ms ? &gev[s] : nullptr));
-    if (measure_plaq) RET(plaquette_async(g, prec, lat, series + 4 * s));
+                   kernel_ms ? &gev[s] : nullptr, measure_plaq == 1 ? series + 4 * s : nullptr));
+    if (measure_plaq == 2) RET(plaquette_async(g, prec, lat, series + 4 * s));
   }
   HIPCHK(hipEventRecord(e1, G.st));
   HIPCHK(hipEventSynchronize(e1));

@@ -89,6 +89,18 @@ __host__ __device__ __forceinline__ void site_of(const RepDesc& r, int32_t id, i
   s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
 }
 
+// Loads and stores at a uniform base plus a 32-bit element index: the byte
+// offset stays 32-bit, so the access compiles to global_load/store with an
+// SGPR base and one VGPR offset.
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, uint32_t idx) {
+  return *(const T*)((const char*)base + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
+  *(T*)((char*)base + idx * (uint32_t)sizeof(T)) = v;
+}
+
 // Update-launch decomposition: workgroup b (relative to the replica's first)
 // covers chunk (b % bpp) of the (z, t) plane b / bpp; thread tid of a
 // workgroup of bdim threads takes even/odd site id = plane * P + chunk * bdim
@@ -209,12 +221,34 @@ struct UMath<float> {
 };
 
 // ------------------------------------------------------- wave64 reductions
-// Butterfly over the 64 lanes of a wavefront (ds_bpermute based shuffles);
-// fixed order, so results are reproducible run to run.
+// DPP move of a binary64 value (two 32-bit halves) with a fixed dpp_ctrl:
+// 0xB1 = quad_perm [1,0,3,2], 0x4E = quad_perm [2,3,0,1],
+// 0x141 = row_half_mirror, 0x140 = row_mirror.  All lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wavefront, the same value in every lane.  Four
+// DPP steps give every lane the sum of its 16-lane row (each step adds two
+// equal-shaped partial sums, so partner lanes hold bit-identical values);
+// the four row sums are then added in row order.  Fixed order: results are
+// reproducible run to run.  Must be called with all 64 lanes active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
 }
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll

@@ -23,6 +23,7 @@ struct PhaseArgs {
   uint32_t xcd;                 // 1: XCD-aware workgroup order (xcd_remap)
   const uint32_t* sweep_dev;    // if set, the sweep index is read here (graph replays)
   unsigned long long* accepts;  // if set, accepted hits are added here
+  double* meas;                 // if set (mu >= 1, single lattice): plaquette rows, 4 doubles per workgroup
 };
 
 enum { KIND_METROPOLIS = 0, KIND_OVERRELAX = 1 };
@@ -37,11 +38,20 @@ hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hip
 
 // measurement launchers (u1_measure.hip)
 // partial sums per workgroup; finish_sums folds them in workgroup order.
+// Plaquette: r carries the plane decomposition of set_phase_blocks(r, block *
+// spt) with r.nblk doubled (even sites, then odd sites); nblk = r.nblk.
 hipError_t launch_plaquette(int prec, const void* lat, const RepDesc& r, double* partials /*4*nblk*/, int nblk,
-                            int block, hipStream_t st);
+                            int block, int spt, hipStream_t st);
 hipError_t launch_polyakov(int prec, const void* lat, const RepDesc& r, double* partials /*2*nblk*/, int nblk,
                            int block, hipStream_t st);
-hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double scale, double* out, hipStream_t st);
+// Fold of nblk rows of nv partial sums, in row order, times scale.  Beyond
+// kFoldRows rows the fold takes two levels; scratch then holds
+// fold_groups(nblk) * nv doubles.
+constexpr int kFoldRows = 512;
+inline int fold_groups(int nblk) { return (nblk + kFoldRows - 1) / kFoldRows; }
+inline size_t fold_scratch(int nblk, int nv) { return (size_t)nv * (size_t)fold_groups(nblk); }
+hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double scale, double* out, double* scratch,
+                              hipStream_t st);
 hipError_t launch_poly_volume(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
                               double step, uint32_t stream, uint32_t c1, uint32_t c3, double* P, int block,
                               hipStream_t st);
@@ -70,7 +80,7 @@ hipError_t launch_ml_final(const double* l4, double* ppspace, const RepDesc& r, 
 
 // batched replica plaquettes: one partial row per workgroup, tagged by replica.
 hipError_t launch_plaquette_batched(int prec, const void* lat_base, const RepDesc* reps, const uint16_t* blockmap,
-                                    double* partials /*4*nblk*/, int nblk, int block, hipStream_t st);
+                                    double* partials /*4*nblk*/, int nblk, int block, int spt, hipStream_t st);
 hipError_t launch_batched_finish(const double* partials, const RepDesc* reps, int nrep, double* out /*4*nrep*/,
                                  hipStream_t st);
 hipError_t launch_increment(uint32_t* counter, hipStream_t st);

@@ -111,41 +111,76 @@ __device__ C64 temporal_value(const Real* __restrict__ lat, const RepDesc& r, in
 
 // plaquette.cu:93-118: per site, sum over mu < nu of 1 - e^{iP}, split into
 // space-space and space-time; P = ((t_mu(x) + t_nu(x+mu)) - t_mu(x+nu)) - t_nu(x).
-template <typename Real, bool BATCHED>
+//
+// Decomposition as the update phases (one chunk of one (z, t) plane per
+// workgroup, uniform z/t, SGPR-based loads, XCD-aware order), over both
+// parities: the first nblk/2 workgroups of a replica take the even sites, the
+// rest the odd sites.  SPT sites per thread.  A site reads its four links and
+// the twelve forward neighbours it needs (16 loads; every link is read by four
+// sites, mostly from L2).  The three plaquettes of each class are added in
+// Real, then once per site into the FP64 accumulators.
+template <typename Real, bool BATCHED, int SPT>
 __global__ __launch_bounds__(256) void k_plaquette(const Real* __restrict__ lat_base, RepDesc single,
                                                    const RepDesc* reps, const uint16_t* blockmap,
                                                    double* __restrict__ partials) {
-  const RepDesc& r = BATCHED ? reps[blockmap[blockIdx.x]] : single;
+  const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+  const RepDesc& r = BATCHED ? reps[blockmap[L]] : single;
   const Real* lat = lat_base + r.lat_off;
-  const int32_t idx = ((int32_t)blockIdx.x - r.blk_begin) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (idx < r.V) {
-    const int32_t parity = idx >= r.HV ? 1 : 0;
-    const int32_t id = idx - parity * r.HV;
-    Site s;
-    site_of(r, id, parity, s);
-    const int32_t same = parity * r.HV + id, opp = (1 - parity) * r.HV + id;
+  uint32_t b = L - (uint32_t)r.blk_begin;
+  const uint32_t half = (uint32_t)r.nblk >> 1;
+  const int32_t parity = b >= half ? 1 : 0;
+  if (parity) b -= half;
+  PhaseSite ps[SPT];
 #pragma unroll
-    for (int mu = 0; mu < 3; ++mu) {
-      const Real tmu = lat[(int64_t)mu * r.V + same];
+  for (int k = 0; k < SPT; ++k) {
+    phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, parity, ps[k]);
+    if (!ps[k].valid) {
+      ps[k].id = 0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) ps[k].fwd[d] = ps[k].bwd[d] = 0;
+    }
+  }
+  Real tm[SPT][4], tf[SPT][4][4];  // tf[k][nu][mu] = theta_nu(x + mu), mu < nu or mu > nu
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    const uint32_t same = (uint32_t)(parity * r.HV) + ps[k].id;
+    const uint32_t opp = (uint32_t)((1 - parity) * r.HV) + ps[k].id;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) tm[k][mu] = ld(lat + (size_t)mu * r.V, same);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu)
+        if (nu != mu) tf[k][nu][mu] = ld(lat + (size_t)nu * r.V, opp + ps[k].fwd[mu]);
+  }
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    Real re[2] = {0, 0}, im[2] = {0, 0};
+#pragma unroll
+    for (int mu = 0; mu < 3; ++mu)
 #pragma unroll
       for (int nu = mu + 1; nu < 4; ++nu) {
-        Real p = tmu;
-        p += lat[(int64_t)nu * r.V + opp + s.fwd[mu]];
-        p -= lat[(int64_t)mu * r.V + opp + s.fwd[nu]];
-        p -= lat[(int64_t)nu * r.V + same];
+        const Real p = ((tm[k][mu] + tf[k][nu][mu]) - tf[k][mu][nu]) - tm[k][nu];
         Real sp, cp;
         Prec<Real>::sincos(p, &sp, &cp);
-        const int k = (nu == 3) ? 2 : 0;
-        v[k] += 1.0 - (double)cp;
-        v[k + 1] += -(double)sp;
+        const int c = nu == 3 ? 1 : 0;
+        re[c] += (Real)1 - cp;
+        im[c] -= sp;
       }
+    if (ps[k].valid) {
+      v[0] += (double)re[0];
+      v[1] += (double)im[0];
+      v[2] += (double)re[1];
+      v[3] += (double)im[1];
     }
   }
   double out[4];
   block_sum<4>(v, out);
+  // row L (the logical workgroup): a batched fold reads each replica's rows
+  // [blk_begin, blk_begin + nblk)
   if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) partials[4 * (int64_t)blockIdx.x + k] = out[k];
+    for (int k = 0; k < 4; ++k) partials[4 * (int64_t)L + k] = out[k];
 }
 
 // polyakov.cu:60-77: sum of the temporal angles along t (in FP64), then e^{i.}.
@@ -183,6 +218,19 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
   double o[1];
   block_sum<1>(v, o);
   if (threadIdx.x == 0) out[k] = o[0] * scale;
+}
+
+// First level of a long fold: workgroup (g, k) adds rows [g R, g R + R) of
+// column k into row g of l1 (same row layout, nv values per row).
+__global__ __launch_bounds__(256) void k_fold_rows(const double* __restrict__ partials, int nrows, int nv, int R,
+                                                   double* __restrict__ l1) {
+  const int g = blockIdx.x, k = blockIdx.y;
+  const int r0 = g * R, r1 = min(nrows, r0 + R);
+  double v[1] = {0.0};
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += blockDim.x) v[0] += partials[(int64_t)i * nv + k];
+  double o[1];
+  block_sum<1>(v, o);
+  if (threadIdx.x == 0) l1[(int64_t)g * nv + k] = o[0];
 }
 
 // polyakov.cu:415-440: P(x) = prod_t T(x, t), stored at the lexicographic
@@ -438,23 +486,36 @@ inline int nblocks(int64_t n, int block) { return (int)((n + block - 1) / block)
 
 }  // namespace
 
-hipError_t launch_plaquette(int prec, const void* lat, const RepDesc& r, double* partials, int nblk, int block,
-                            hipStream_t st) {
-  if (prec == PREC_F64)
-    k_plaquette<double, false><<<nblk, block, 0, st>>>((const double*)lat, r, nullptr, nullptr, partials);
+namespace {
+template <typename Real, bool BATCHED>
+hipError_t plaquette_spt(int spt, const Real* lat, const RepDesc& r, const RepDesc* reps, const uint16_t* map,
+                         double* partials, int nblk, int block, hipStream_t st) {
+  if (spt == 4 && sizeof(Real) == sizeof(float))
+    k_plaquette<Real, BATCHED, 4><<<nblk, block, 0, st>>>(lat, r, reps, map, partials);
+  else if (spt == 2)
+    k_plaquette<Real, BATCHED, 2><<<nblk, block, 0, st>>>(lat, r, reps, map, partials);
+  else if (spt == 1)
+    k_plaquette<Real, BATCHED, 1><<<nblk, block, 0, st>>>(lat, r, reps, map, partials);
   else
-    k_plaquette<float, false><<<nblk, block, 0, st>>>((const float*)lat, r, nullptr, nullptr, partials);
+    return hipErrorInvalidValue;
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_plaquette(int prec, const void* lat, const RepDesc& r, double* partials, int nblk, int block,
+                            int spt, hipStream_t st) {
+  if (prec == PREC_F64)
+    return plaquette_spt<double, false>(spt, (const double*)lat, r, nullptr, nullptr, partials, nblk, block, st);
+  return plaquette_spt<float, false>(spt, (const float*)lat, r, nullptr, nullptr, partials, nblk, block, st);
 }
 
 hipError_t launch_plaquette_batched(int prec, const void* lat_base, const RepDesc* reps, const uint16_t* blockmap,
-                                    double* partials, int nblk, int block, hipStream_t st) {
+                                    double* partials, int nblk, int block, int spt, hipStream_t st) {
   RepDesc dummy{};
   if (prec == PREC_F64)
-    k_plaquette<double, true><<<nblk, block, 0, st>>>((const double*)lat_base, dummy, reps, blockmap, partials);
-  else
-    k_plaquette<float, true><<<nblk, block, 0, st>>>((const float*)lat_base, dummy, reps, blockmap, partials);
-  return hipGetLastError();
+    return plaquette_spt<double, true>(spt, (const double*)lat_base, dummy, reps, blockmap, partials, nblk, block,
+                                       st);
+  return plaquette_spt<float, true>(spt, (const float*)lat_base, dummy, reps, blockmap, partials, nblk, block, st);
 }
 
 hipError_t launch_batched_finish(const double* partials, const RepDesc* reps, int nrep, double* out, hipStream_t st) {
@@ -471,8 +532,15 @@ hipError_t launch_polyakov(int prec, const void* lat, const RepDesc& r, double* 
   return hipGetLastError();
 }
 
-hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double scale, double* out, hipStream_t st) {
-  k_finish<<<nv, 256, 0, st>>>(partials, nblk, nv, scale, out);
+hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double scale, double* out, double* scratch,
+                              hipStream_t st) {
+  if (nblk <= kFoldRows) {
+    k_finish<<<nv, 256, 0, st>>>(partials, nblk, nv, scale, out);
+    return hipGetLastError();
+  }
+  const int G = fold_groups(nblk);
+  k_fold_rows<<<dim3(G, nv), 256, 0, st>>>(partials, nblk, nv, kFoldRows, scratch);
+  k_finish<<<nv, 256, 0, st>>>(scratch, G, nv, scale, out);
   return hipGetLastError();
 }
 

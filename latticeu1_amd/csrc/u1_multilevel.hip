@@ -16,7 +16,7 @@ struct C64 {
   double re, im;
 };
 __device__ __forceinline__ C64 cmul(C64 a, C64 b) { return {a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im}; }
-__device__ __forceinline__ C64 ld(const double* p, int64_t i) { return {p[2 * i], p[2 * i + 1]}; }
+__device__ __forceinline__ C64 ldc(const double* p, int64_t i) { return {p[2 * i], p[2 * i + 1]}; }
 
 // multilevel.cu:331-360: l2[x,R,dir,layer] += prod_{t in layer} P(x,t) *
 // prod_{t in layer} conj P(x + R e_dir, t); one thread per (x, R, dir).
@@ -37,8 +37,8 @@ __global__ __launch_bounds__(256) void k_ml_l2(const double* __restrict__ tl, do
   for (int t = 0; t < r.N[3]; t += 2) {
     C64 p0 = {1.0, 0.0}, p1 = {1.0, 0.0};
     for (int tt = t; tt < t + 2; ++tt) {
-      p0 = cmul(p0, ld(tl, s0 + (int64_t)tt * r.SV));
-      const C64 q = ld(tl, s1 + (int64_t)tt * r.SV);
+      p0 = cmul(p0, ldc(tl, s0 + (int64_t)tt * r.SV));
+      const C64 q = ldc(tl, s1 + (int64_t)tt * r.SV);
       p1 = cmul(p1, C64{q.re, -q.im});
     }
     const C64 pl = cmul(p0, p1);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_ml_l4(const double* __restrict__ l2, do
   for (int a = 0; a < nl2; a += 2) {
     C64 pl = {1.0, 0.0};
     for (int layer = a; layer < a + 2; ++layer) {
-      const C64 v = ld(l2, idx + sz * layer);
+      const C64 v = ldc(l2, idx + sz * layer);
       pl = cmul(pl, C64{v.re * l2norm, v.im * l2norm});
     }
     const int64_t pos = idx + sz * j4;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_ml_final(const double* __restrict__ l4,
         C64 pl = {1.0, 0.0};
         const int64_t base = id + (int64_t)r.SV * R + (int64_t)r.SV * rmax * dir;
         for (int j = 0; j < nl4; ++j) {
-          const C64 q = ld(l4, base + sz * j);
+          const C64 q = ldc(l4, base + sz * j);
           pl = cmul(pl, C64{q.re * l4norm, q.im * l4norm});
         }
         v[0] += pl.re;

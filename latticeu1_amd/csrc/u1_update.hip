@@ -29,26 +29,28 @@ namespace u1d {
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ T ld(const T* base, uint32_t idx) {
-  return *(const T*)((const char*)base + idx * (uint32_t)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
-  *(T*)((char*)base + idx * (uint32_t)sizeof(T)) = v;
-}
-
 template <typename Real>
 struct Staple {
   Real re, im;
 };
 
+// Plaquette pieces of a link (x, MU) for the directions nu < MU: the sums of
+// e^{i up_nu} (u) and e^{i lo_nu} (l).  With the link's new angle theta,
+// theta + up_nu = -P_{nu,MU}(x) and theta + lo_nu = P_{nu,MU}(x - nu), so
+// every plaquette of the planes (nu, MU), nu < MU, is one piece of exactly one
+// odd-site MU link.
+template <typename Real>
+struct PlaqPieces {
+  Real ur, ui, lr, li;
+};
+
 // Staple of link (x, MU) (staple.h:79-102), with the operation order of the
-// oracle.  fwd/bwd[d] are the eo-id offsets to x +- e_d.
-template <typename Real, int MU>
+// oracle.  fwd/bwd[d] are the eo-id offsets to x +- e_d.  With MEAS the
+// plaquette pieces are collected on the side (the staple is unchanged).
+template <typename Real, int MU, bool MEAS = false>
 __device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, const RepDesc& r, uint32_t same,
                                                   uint32_t opp, const int32_t (&fwd)[4], const int32_t (&bwd)[4],
-                                                  Real xi, Real inv_xi, bool aniso) {
+                                                  Real xi, Real inv_xi, bool aniso, PlaqPieces<Real>* pp = nullptr) {
   Real ssr = 0, ssi = 0, str = 0, sti = 0;
   const Real* pm = lat + (size_t)MU * r.V;
 #pragma unroll
@@ -67,6 +69,12 @@ __device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, 
     UMath<Real>::sincos(up, &su, &cu);
     UMath<Real>::sincos(lo, &sl, &cl);
     const Real ar = cu + cl, ai = su + sl;
+    if (MEAS && nu < MU) {
+      pp->ur += cu;
+      pp->ui += su;
+      pp->lr += cl;
+      pp->li += sl;
+    }
     if (MU == 3 || nu == 3) {
       str += ar;
       sti += ai;
@@ -93,12 +101,14 @@ struct MetroState {
 template <typename Real>
 __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, Real beta, int proposal, double step,
                                           uint32_t wu, uint32_t wb) {
-  const double u = u1r::u01_d(wu);
+  // a * (2u - 1) with u = wu 2^-32 as one FMA: 2u - 1 is exact in binary64,
+  // so fma(wu, a 2^-31, -a) is the same correctly rounded product as the
+  // oracle's two-step form (a 2^-31 is exact).
   double tnd;
   if (proposal == 0) {
-    tnd = u1m::K_PI * (2.0 * u - 1.0);
+    tnd = __builtin_fma((double)wu, u1m::K_PI * 0x1p-31, -u1m::K_PI);
   } else {
-    tnd = (double)m.th + step * (2.0 * u - 1.0);
+    tnd = (double)m.th + __builtin_fma((double)wu, step * 0x1p-31, -step);
     if (tnd >= u1m::K_PI) tnd -= u1m::K_TWO_PI;
     if (tnd < -u1m::K_PI) tnd += u1m::K_TWO_PI;
   }
@@ -119,7 +129,14 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // of one (z, t) plane.  All angle loads of all SPT sites are issued before the
 // transcendental work, so each wave keeps SPT x 19 loads in flight.  Threads
 // past the end of a plane are pointed at site 0 (in bounds) and do not store.
-template <typename Real, int MU, int KIND, bool BATCHED, int SPT>
+//
+// MEAS (single lattice, MU >= 1, parity 1 of the last pass of a sweep): the
+// workgroup also writes row L of a.meas with its sums over the plaquettes of
+// the planes (nu, MU), nu < MU, after the update -- [sum (1 - cos P), sum
+// -sin P] in the space-space (MU < 3) or space-time (MU = 3) slot.  The three
+// phases (1, 1), (1, 2), (1, 3) come after every link of those planes has
+// reached its final value, and together cover all six planes once.
+template <typename Real, int MU, int KIND, bool BATCHED, int SPT, bool MEAS = false>
 __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, PhaseArgs a) {
   const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
@@ -136,6 +153,7 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   uint32_t same[SPT];
   Staple<Real> S[SPT];
   Real th[SPT];
+  PlaqPieces<Real> pp[SPT];
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {
     if (!ps[k].valid) {
@@ -145,8 +163,9 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     }
     same[k] = (uint32_t)(a.parity * r.HV) + ps[k].id;
     const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
-    S[k] = staple_at<Real, MU>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, (Real)r.aniso, (Real)(1.0 / r.aniso),
-                               aniso);
+    pp[k] = {0, 0, 0, 0};
+    S[k] = staple_at<Real, MU, MEAS>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, (Real)r.aniso,
+                                     (Real)(1.0 / r.aniso), aniso, &pp[k]);
     th[k] = ld(pmu, same[k]);
   }
   int acc = 0;
@@ -189,6 +208,31 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 #pragma unroll
   for (int k = 0; k < SPT; ++k)
     if (ps[k].valid) st(pmu, same[k], th[k]);
+  if (MEAS) {
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      Real sn, cn;
+      UMath<Real>::sincos(th[k], &sn, &cn);
+      // sum cos(theta + up) + cos(theta + lo); sum sin(theta + up) - sin(theta + lo)
+      const Real re = cn * (pp[k].ur + pp[k].lr) - sn * (pp[k].ui + pp[k].li);
+      const Real im = sn * (pp[k].ur - pp[k].lr) + cn * (pp[k].ui - pp[k].li);
+      if (ps[k].valid) {
+        v[0] += (double)(2 * MU) - (double)re;
+        v[1] += (double)im;
+      }
+    }
+    double o[2];
+    block_sum<2>(v, o);
+    if (threadIdx.x == 0) {
+      const int c = MU == 3 ? 2 : 0;
+      double* row = a.meas + 4 * (size_t)L;
+      row[c] = o[0];
+      row[c + 1] = o[1];
+      row[2 - c] = 0.0;
+      row[3 - c] = 0.0;
+    }
+  }
   if (KIND == KIND_METROPOLIS && a.accepts) {
     const uint64_t tot = wave_sum_u64((uint64_t)acc);
     if ((threadIdx.x & 63) == 0 && tot) atomicAdd(a.accepts, (unsigned long long)tot);
@@ -205,6 +249,16 @@ __global__ __launch_bounds__(256) void k_hotstart(Real* __restrict__ lat, RepDes
 
 template <typename Real, int KIND, bool BATCHED, int SPT>
 hipError_t launch_mu(int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+  if (a.meas != nullptr) {
+    // measuring phases: single lattice, mu >= 1, no multilevel mask
+    if (BATCHED || mu < 1 || a.ml != 0) return hipErrorInvalidValue;
+    switch (mu) {
+      case 1: k_phase<Real, 1, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+      case 2: k_phase<Real, 2, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+      default: k_phase<Real, 3, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+    }
+    return hipGetLastError();
+  }
   switch (mu) {
     case 0: k_phase<Real, 0, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;
     case 1: k_phase<Real, 1, KIND, BATCHED, SPT><<<nblocks, block, 0, s>>>(lat, a); break;

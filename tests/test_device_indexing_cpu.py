@@ -24,3 +24,18 @@ def test_device_indexing_on_cpu(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "indexing ok" in out.stdout
+
+
+def test_fma_proposal_matches_two_step_form(tmp_path):
+    """The kernels' one-FMA Metropolis proposal offset is bit-identical to the
+    oracle's two-step a (2u - 1) (tests/cpu_checks/check_proposal.cpp)."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "check_proposal"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-mfma",
+                    os.path.join(HERE, "cpu_checks", "check_proposal.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "proposal ok" in out.stdout

@@ -135,6 +135,65 @@ def test_fp64_parity_geometries(u1, orc, dims, beta, aniso, metrop, ovrn, nsw):
     assert np.max(np.abs(lat.plaquette() - orc.plaquette(og, ref))) <= 1e-12
 
 
+@pytest.mark.parametrize("dims", [
+    (2, 2, 2, 2),      # P = 2: one mostly idle workgroup per plane
+    (10, 6, 4, 6),     # P = 30, ragged extents
+    (16, 16, 4, 4),    # P = 128: exact chunks
+    (24, 20, 4, 4),    # P = 240: tail chunk with idle threads
+    (48, 48, 4, 4),    # P = 1152: two sites per thread
+    (64, 32, 2, 2),    # P = 1024: four sites per thread (binary32)
+])
+def test_plaquette_decompositions(u1, orc, dims):
+    """Plaquette over both parities with the plane-chunk decomposition: FP64
+    within 1e-12 of the oracle, FP32 links within 1e-6 relative."""
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=5)
+    say(f"plaquette decomposition {dims}")
+    p64 = orc.plaquette(og, ref)
+    assert np.max(np.abs(lat.plaquette() - p64)) <= 1e-12
+    p32 = lat.astype(u1.F32).plaquette()
+    for k in (0, 2):
+        assert abs(p32[k] - p64[k]) <= 1e-6 * abs(p64[k])
+    for k in (1, 3):  # imaginary parts: O(1/sqrt(V)) for a hot start
+        assert abs(p32[k] - p64[k]) <= 1e-6
+
+
+@pytest.mark.parametrize("dims,aniso,metrop,ovrn", [
+    ((8, 8, 8, 8), 1.0, 1, 3),
+    ((6, 10, 4, 12), 1.0, 2, 0),   # Metropolis-only sweep: the last Metropolis pass measures
+    ((4, 4, 4, 8), 2.0, 1, 2),     # anisotropic: plaquettes are unweighted
+    ((2, 2, 2, 2), 1.0, 1, 1),
+])
+def test_fused_plaquette_fp64(u1, orc, dims, aniso, metrop, ovrn):
+    """update_plaquette: the same links as update(), and the plaquette of the
+    result within 1e-13 of the oracle's."""
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, aniso, True, seed=9)
+    say(f"fused plaquette fp64 {dims} xi={aniso} metrop={metrop} ovrn={ovrn}")
+    m = u1.BASELINE_METRO
+    for s in range(3):
+        p = lat.update_plaquette(metrop, ovrn, m, 9, s)
+        orc.update(og, ref, metrop, ovrn, orc.metro(1, 4, 0.5), 9, s)
+        assert np.array_equal(bits(lat.to_host()), bits(ref))
+        assert np.max(np.abs(p - orc.plaquette(og, ref))) <= 1e-13, (p, orc.plaquette(og, ref))
+
+
+def test_fused_plaquette_fp32():
+    """FP32: the fused plaquette (hardware sin/cos pieces) against the
+    separate pass on the same updated links, 48x48x4x4 and 16^4."""
+    import latticeu1_amd as u1
+
+    for dims in ((48, 48, 4, 4), (16, 16, 16, 16)):
+        g = u1.geom(dims, 1.0)
+        lat = u1.Lattice.cold(g, u1.F32)
+        lat.hotstart(21)
+        for s in range(3):
+            p = lat.update_plaquette(1, 3, u1.BASELINE_METRO, 21, s)
+            q = lat.plaquette()
+            for k in (0, 2):
+                assert abs(p[k] - q[k]) <= 2e-6 * abs(q[k]), (dims, s, p, q)
+            for k in (1, 3):
+                assert abs(p[k] - q[k]) <= 2e-6, (dims, s, p, q)
+
+
 def test_phase_write_set_gpu(u1, orc):
     """Each (parity, mu) launch modifies only its own half-plane."""
     dims = (8, 4, 6, 4)
