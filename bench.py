@@ -17,7 +17,9 @@ The JSON line carries:
   value          FP32-bulk link updates/s (whole timed region, inputs resident)
   roofline       dominant FP32 phase kernel: algorithmic bytes per launch
                  (9 * 4 B per link update * V/2 links) / mean launch duration
-                 from HIP events on the library stream, against 8 TB/s HBM
+                 from HIP events on the library stream, against 8 TB/s HBM;
+                 traffic (HBM bytes per launch) from rocprofv3 FETCH_SIZE /
+                 WRITE_SIZE counter files when given with --traffic, else null
   cpu_baseline   the FP64 host oracle (OpenMP) on a bounded sample of the
                  same sweeps, timed on this host's cores
   fp64 / ratio / plaquette errors as extra fields.
@@ -92,6 +94,37 @@ def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
     return lat, ms, kernel_ms, series
 
 
+def counter_traffic(fetch_csv, write_csv, kind):
+    """HBM bytes per launch of the FP32 phase kernel of `kind` (0 Metropolis,
+    1 overrelaxation; the plain variant, not the measuring one) from two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each
+    scaled by what the same counter reads for the plain copy kernel
+    k_convert_precision<float, float> (Grid_Size elements of 4 B read and 4 B
+    written) in the same run.  Returns (traffic bytes, fetch factor, write factor)."""
+    import csv
+    import re
+
+    pat = re.compile(r"k_phase<float, \d, %d, false, \d, false>" % kind)
+
+    def scan(path):
+        vals, calib = [], []
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name = row["Kernel_Name"]
+                if pat.search(name):
+                    vals.append(float(row["Counter_Value"]))
+                elif "k_convert_precision<float, float>" in name:
+                    calib.append((float(row["Counter_Value"]), 4.0 * float(row["Grid_Size"])))
+        if not vals or not calib:
+            raise ValueError(f"{path}: no phase-kernel or calibration dispatches")
+        kib, nbytes = calib[-1]
+        return float(np.mean(vals)), kib * 1024.0 / nbytes
+
+    fetch_kib, ff = scan(fetch_csv)
+    write_kib, fw = scan(write_csv)
+    return fetch_kib * 1024.0 / ff + write_kib * 1024.0 / fw, ff, fw
+
+
 def cpu_baseline(dims, beta, seed, budget_s):
     """FP64 host oracle (the timed CPU baseline), bounded to about budget_s."""
     from oracle.oracle import Oracle
@@ -130,6 +163,9 @@ def main():
     ap.add_argument("--tunecache", type=str, default="",
                     help="launch-shape cache: loaded if the file exists (no tuning launches), else written after "
                          "tuning")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
+                    help="counter_collection.csv files of earlier rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs "
+                         "of this script (with --calib-mb): fills roofline.traffic")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the run, copy a buffer of this many MiB once (plain copy kernel of known size, "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -166,12 +202,15 @@ def main():
         res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, plaq=mean,
                          plaq_err=err, tau=tau, bytes=bytes_per_launch, elem=s)
         print(f"[bench] {name}: {res[name]['rate']:.4e} link updates/s, {ms / args.steps:.3f} ms/sweep, "
-              f"<P> = {mean:.8f} +- {err:.1e}, mean phase kernel ms M/OR = {kms[0]:.4f}/{kms[1]:.4f}",
-              file=sys.stderr, flush=True)
+              f"<P> = {mean:.8f} +- {err:.1e}, mean phase kernel ms M/OR/OR+plaquette = "
+              f"{kms[0]:.4f}/{kms[1]:.4f}/{kms[2]:.4f}", file=sys.stderr, flush=True)
 
     f32 = res["fp32"]
-    # dominant kernel class: the one with the larger total time per sweep
-    t_m, t_o = 8 * f32["kms"][0], 24 * f32["kms"][1]
+    # dominant kernel class: the one with the larger total time per sweep; with
+    # the fused plaquette, 3 of the 24 OR launches are the measuring variant
+    n_meas = 3 if args.plaq == "fused" else 0
+    t_m = 8 * f32["kms"][0]
+    t_o = (24 - n_meas) * f32["kms"][1] + n_meas * f32["kms"][2]
     dom = "overrelaxation" if t_o >= t_m else "metropolis"
     dom_ms = f32["kms"][1] if dom == "overrelaxation" else f32["kms"][0]
     achieved = f32["bytes"] / (dom_ms * 1e-3) / 1e9
@@ -179,21 +218,30 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": f"k_phase<float> {dom} (parity, mu) phase",
                 "bytes_per_launch": f32["bytes"], "mean_launch_ms": round(dom_ms, 5),
-                "other_kernel": {"metropolis_ms": round(f32["kms"][0], 5), "overrelaxation_ms": round(f32["kms"][1], 5)},
+                "other_kernel": {"metropolis_ms": round(f32["kms"][0], 5),
+                                 "overrelaxation_ms": round(f32["kms"][1], 5),
+                                 "overrelaxation_with_plaquette_ms": round(f32["kms"][2], 5)},
                 "sweep_algorithmic_GBs": round(32 * f32["bytes"] / ((t_m + t_o) * 1e-3) / 1e9, 1)}
+
+    if args.traffic:
+        tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], 1 if dom == "overrelaxation" else 0)
+        roofline["traffic"] = round(tb)
+        roofline["traffic_over_algorithmic"] = round(tb / f32["bytes"], 3)
+        roofline["traffic_calibration"] = {"fetch_factor": round(ff, 4), "write_factor": round(fw, 4)}
 
     extra = {}
     if "fp64" in res:
         f64 = res["fp64"]
         extra["fp64_link_updates_per_s"] = f64["rate"]
         extra["fp32_over_fp64"] = f32["rate"] / f64["rate"]
-        extra["fp64_kernel_ms"] = {"metropolis": round(f64["kms"][0], 5), "overrelaxation": round(f64["kms"][1], 5)}
+        extra["fp64_kernel_ms"] = {"metropolis": round(f64["kms"][0], 5), "overrelaxation": round(f64["kms"][1], 5),
+                                   "overrelaxation_with_plaquette": round(f64["kms"][2], 5)}
         d = f32["plaq"] - f64["plaq"]
         comb = math.hypot(f32["plaq_err"], f64["plaq_err"])
         extra["plaquette"] = {"fp32": [f32["plaq"], f32["plaq_err"]], "fp64": [f64["plaq"], f64["plaq_err"]],
                               "tau_int_sweeps": {"fp32": f32["tau"], "fp64": f64["tau"]},
                               "diff": d, "diff_over_combined_err": d / comb if comb > 0 else None,
-                              "within_1sigma_of_fp64": abs(d) <= f64["plaq_err"]}
+                              "within_1sigma": abs(d) <= comb}
         # fixed configuration: FP32 measurement of the final FP64 configuration
         p64 = f64["lat"].plaquette()
         p32 = f64["lat"].astype(u1.F32).plaquette()

@@ -200,9 +200,11 @@ int u1_tune_entries(int32_t* n);
  * measurement into plaq_series[4*i] after every sweep if measure_plaq: 1 fused
  * into the sweep as u1_update_plaquette, 2 a separate u1_plaquette pass) and
  * return the elapsed device time in ms, from HIP events on the library stream.
- * kernel_ms (nullable): mean launch duration of the Metropolis (kernel_ms[0])
- * and overrelaxation (kernel_ms[1]) phase kernels, from events that bracket
- * each sweep's back-to-back launch groups inside the same timed region. */
+ * kernel_ms (nullable, 3 doubles): mean launch duration of the Metropolis
+ * phases (kernel_ms[0]), the overrelaxation phases without measurement
+ * (kernel_ms[1]) and, with measure_plaq 1 and ovrn > 0, the three measuring
+ * overrelaxation phases (kernel_ms[2], else 0), from events that bracket each
+ * sweep's back-to-back launch groups inside the same timed region. */
 int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms);

@@ -353,7 +353,7 @@ class Lattice:
         measure_plaq = int(measure_plaq)
         series = np.zeros(4 * nsweeps) if measure_plaq else None
         el = C.c_double(0)
-        km = np.zeros(2)
+        km = np.zeros(3)
         call("u1_bench_updates", C.byref(self.g), self.prec, self.buf.ptr, metrop, ovrn, C.byref(m), seed, sweep0,
              nsweeps, measure_plaq, _dp(series) if measure_plaq else None, C.byref(el), _dp(km))
         return el.value, km, (series.reshape(nsweeps, 4) if measure_plaq else None)

@@ -250,8 +250,12 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
 
 // Events bracketing the Metropolis and overrelaxation groups of one sweep
 // (for the mean launch duration of each kernel class).
+// Events around one sweep's launch groups: start | Metropolis | after_metro |
+// OR phases without measurement | before_meas | measuring phases (and the OR
+// phases between them) | after_or.  Without measurement before_meas is
+// recorded together with after_or.
 struct GroupEvents {
-  hipEvent_t start, after_metro, after_or;
+  hipEvent_t start, after_metro, before_meas, after_or;
 };
 
 // One UpdateLattice on a single lattice (update.cu:449-456 loop order).
@@ -301,14 +305,20 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
   a.single = ro;
   a.xcd = (uint32_t)co.xcd;
+  bool marked = false;
   for (int o = 0; o < ovrn; ++o)
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
         a.meas = meas_rows(o == ovrn - 1, parity, mu);
+        if (gev && a.meas && !marked) {
+          HIPCHK(hipEventRecord(gev->before_meas, G.st));
+          marked = true;
+        }
         HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, ro.nblk, co.block, co.spt, G.st));
       }
+  if (gev && !marked) HIPCHK(hipEventRecord(gev->before_meas, G.st));
   if (gev) HIPCHK(hipEventRecord(gev->after_or, G.st));
   if (meas_dst)
     HIPCHK(u1d::launch_finish_sums(G.partials, 3 * meas_nblk, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
@@ -1159,6 +1169,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   for (auto& e : gev) {
     HIPCHK(hipEventCreate(&e.start));
     HIPCHK(hipEventCreate(&e.after_metro));
+    HIPCHK(hipEventCreate(&e.before_meas));
     HIPCHK(hipEventCreate(&e.after_or));
   }
   hipEvent_t e0, e1;
@@ -1177,19 +1188,27 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *elapsed_ms = ms;
   if (kernel_ms) {
-    double tm = 0, to = 0;
+    // measuring OR phases: (parity 1, mu 1..3) of the last pass, i.e. the last
+    // three OR launches of a sweep
+    const bool meas_or = measure_plaq == 1 && ovrn > 0;
+    const int n_or_plain = 8 * ovrn - (meas_or ? 3 : 0);
+    double tm = 0, to = 0, tq = 0;
     for (auto& e : gev) {
-      float a = 0, b = 0;
+      float a = 0, b = 0, c = 0;
       HIPCHK(hipEventElapsedTime(&a, e.start, e.after_metro));
-      HIPCHK(hipEventElapsedTime(&b, e.after_metro, e.after_or));
+      HIPCHK(hipEventElapsedTime(&b, e.after_metro, e.before_meas));
+      HIPCHK(hipEventElapsedTime(&c, e.before_meas, e.after_or));
       tm += a;
       to += b;
+      tq += c;
       HIPCHK(hipEventDestroy(e.start));
       HIPCHK(hipEventDestroy(e.after_metro));
+      HIPCHK(hipEventDestroy(e.before_meas));
       HIPCHK(hipEventDestroy(e.after_or));
     }
     kernel_ms[0] = metrop > 0 ? tm / (8.0 * metrop * nsweeps) : 0.0;
-    kernel_ms[1] = ovrn > 0 ? to / (8.0 * ovrn * nsweeps) : 0.0;
+    kernel_ms[1] = n_or_plain > 0 ? to / ((double)n_or_plain * nsweeps) : 0.0;
+    kernel_ms[2] = meas_or ? tq / (3.0 * nsweeps) : 0.0;
   }
   if (measure_plaq && plaq_series)
     HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));
