@@ -19,7 +19,9 @@ The JSON line carries:
                  (9 * 4 B per link update * V/2 links) / mean launch duration
                  from HIP events on the library stream, against 8 TB/s HBM;
                  traffic (HBM bytes per launch) from rocprofv3 FETCH_SIZE /
-                 WRITE_SIZE counter files when given with --traffic, else null
+                 WRITE_SIZE counter files when given with --traffic, else null;
+                 stream_reference_GBs: plain streaming kernels moving the same
+                 bytes (read only, copy, and the phase's 8:1 read:write mix)
   cpu_baseline   the FP64 host oracle (OpenMP) on a bounded sample of the
                  same sweeps, timed on this host's cores
   fp64 / ratio / plaquette errors as extra fields.
@@ -87,7 +89,7 @@ def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
     g = u1.geom(dims, beta, 1.0)
     lat = u1.Lattice.cold(g, prec)
     lat.hotstart(seed)
-    m = u1.BASELINE_METRO
+    m = u1.BENCH_METRO
     if warmup > 0:
         lat.bench(1, 3, m, seed, 0, warmup, 0)  # thermalisation (also runs the autotuner)
     ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, plaq_mode)
@@ -172,7 +174,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding along t is described in DESIGN.md but not built)", file=sys.stderr)
+              "(multi-GPU sharding along t is designed but not built)", file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
     V = int(np.prod(dims))
@@ -222,6 +224,13 @@ def main():
                                  "overrelaxation_ms": round(f32["kms"][1], 5),
                                  "overrelaxation_with_plaquette_ms": round(f32["kms"][2], 5)},
                 "sweep_algorithmic_GBs": round(32 * f32["bytes"] / ((t_m + t_o) * 1e-3) / 1e9, 1)}
+
+    # streaming references moving the same 36 B x V/2 bytes as one FP32 phase
+    # with an ideal pattern: what this HBM delivers in practice
+    n = V // 2 - (V // 2) % 8
+    ref = {"read": u1.stream_rate(0, n), "copy": u1.stream_rate(1, n), "read8_write1": u1.stream_rate(2, n)}
+    roofline["stream_reference_GBs"] = {k: round(v, 1) for k, v in ref.items()}
+    roofline["frac_of_read8_write1_stream"] = round(achieved / ref["read8_write1"], 4)
 
     if args.traffic:
         tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], 1 if dom == "overrelaxation" else 0)

@@ -220,6 +220,13 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
 int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n);
 /* Philox4x32-10 on the device: ctr[4n], key[2n] -> out[4n] (host arrays). */
 int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n);
+/* Streaming reference for the roofline (no counterpart in the reference):
+ * moves the bytes of one (parity, mu) phase, 36 n (4.5 floats per each of n
+ * updated links), with an ideal access pattern and returns the rate in GB/s
+ * averaged over reps launches.  pattern 0: read 9n floats; 1: copy 4.5n
+ * floats; 2: read eight n-float arrays, write one (the phase's 8:1 mix).
+ * n % 8 == 0, n <= 2^30. */
+int u1_selftest_stream(int32_t pattern, int64_t n, int32_t reps, double* gbs);
 
 #ifdef __cplusplus
 }

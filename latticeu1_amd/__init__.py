@@ -60,7 +60,7 @@ def metro(proposal: int = PROPOSAL_INDEPENDENT, nhit: int = 1, step: float = 0.5
 
 
 REFERENCE_METRO = metro(PROPOSAL_INDEPENDENT, 1, 0.0)  # update.cu:181-182
-BASELINE_METRO = metro(PROPOSAL_DELTA, 4, 0.5)  # theta + U[-0.5, 0.5), 4 hits
+BENCH_METRO = metro(PROPOSAL_DELTA, 4, 0.5)  # theta + U[-0.5, 0.5), 4 hits
 
 _P = C.c_void_p
 _SIGS = {
@@ -82,6 +82,7 @@ _SIGS = {
                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32),
     "u1_selftest_math": (C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64),
     "u1_selftest_philox": (C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int64),
+    "u1_selftest_stream": (C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)),
     "u1_plaquette": (C.POINTER(Geom), C.c_int, _P, C.POINTER(C.c_double)),
     "u1_polyakov": (C.POINTER(Geom), C.c_int, _P, C.POINTER(C.c_double)),
     "u1_poly_volume": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.POINTER(Metro), C.c_uint64, C.c_uint32, C.c_uint32,
@@ -448,3 +449,10 @@ def tune_load(path: str) -> None:
 def mean_plaquette(p4: np.ndarray) -> float:
     """<P> = 1 - Re(plaq_SS + plaq_ST)/2 (the drop-in return convention is 1 - <e^{iP}>)."""
     return 1.0 - 0.5 * (p4[0] + p4[2])
+
+
+def stream_rate(pattern: int, n: int, reps: int = 20) -> float:
+    """GB/s of the streaming reference moving 36 n bytes (u1_selftest_stream)."""
+    out = C.c_double(0)
+    call("u1_selftest_stream", pattern, n, reps, C.byref(out))
+    return out.value

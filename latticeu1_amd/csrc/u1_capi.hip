@@ -1239,6 +1239,42 @@ int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, 
   return U1_OK;
 }
 
+int u1_selftest_stream(int32_t pattern, int64_t n, int32_t reps, double* gbs) {
+  REQ_INIT();
+  if (pattern < 0 || pattern > 2 || n < 8 || n % 8 != 0 || n > ((int64_t)1 << 30) || reps < 1 || !gbs)
+    return fail(U1_ERR_ARG, "bad stream arguments");
+  float *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc((void**)&a, 9 * (size_t)n * sizeof(float)));
+  int rc = U1_OK;
+  if (hipMalloc((void**)&b, 9 * (size_t)n * sizeof(float)) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(U1_ERR_HIP, "hipMalloc failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto run = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = u1d::launch_fill_zero(a, 36 * n, G.st)) != hipSuccess) return e;
+    if ((e = u1d::launch_stream(pattern, a, b, n, G.st)) != hipSuccess) return e;  // warm-up
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return e;
+    if ((e = hipEventRecord(e0, G.st)) != hipSuccess) return e;
+    for (int i = 0; i < reps; ++i)
+      if ((e = u1d::launch_stream(pattern, a, b, n, G.st)) != hipSuccess) return e;
+    if ((e = hipEventRecord(e1, G.st)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+    float ms = 0;
+    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return e;
+    *gbs = 36.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    return hipSuccess;
+  };
+  const hipError_t e = run();
+  if (e != hipSuccess) rc = fail(U1_ERR_HIP, std::string("stream probe: ") + hipGetErrorString(e));
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
 int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n) {
   REQ_INIT();
   if (!ctr || !key || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");

@@ -89,5 +89,7 @@ hipError_t launch_increment(uint32_t* counter, hipStream_t st);
 hipError_t launch_selftest_math(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t st);
 hipError_t launch_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n,
                                   hipStream_t st);
+// streaming reference over 36 n bytes (a: 9n floats, b: n floats; n % 8 == 0)
+hipError_t launch_stream(int pattern, float* a, float* b, int64_t n, hipStream_t st);
 
 }  // namespace u1d

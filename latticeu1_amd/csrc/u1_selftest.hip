@@ -64,6 +64,45 @@ hipError_t launch_selftest_math(int fn, const double* x, const double* y, double
   return hipGetLastError();
 }
 
+// Streaming references for the roofline: the same bytes as one (parity, mu)
+// phase with an ideal access pattern.  Pattern 0 reads 9n floats (as float4),
+// pattern 1 copies 4.5n floats (float4), pattern 2 reads eight arrays of n
+// floats and writes one (the phase's 8:1 mix, fully coalesced, no stencil).
+__global__ __launch_bounds__(256) void k_stream_read(const float4* __restrict__ a, int64_t n4, float* __restrict__ sink) {
+  float s = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 v = a[i];
+    s += (v.x + v.y) + (v.z + v.w);
+  }
+  if (s == -1.0f) sink[0] = s;  // keeps the loads; the data are zeros, so never taken
+}
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, int64_t n4) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+__global__ __launch_bounds__(256) void k_stream_r8w1(const float* __restrict__ a, float* __restrict__ b, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[i + k * n];
+  b[i] = s;
+}
+
+hipError_t launch_stream(int pattern, float* a, float* b, int64_t n, hipStream_t st) {
+  const int grid = 4096;  // grid-stride: 16 workgroups per CU
+  if (pattern == 0) {
+    k_stream_read<<<grid, 256, 0, st>>>((const float4*)a, 9 * n / 4, b);
+  } else if (pattern == 1) {
+    k_stream_copy<<<grid, 256, 0, st>>>((const float4*)a, (float4*)b, 9 * n / 8);
+  } else if (pattern == 2) {
+    k_stream_r8w1<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, b, n);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n,
                                   hipStream_t st) {
   k_philox<<<(int)((n + 255) / 256), 256, 0, st>>>(ctr, key, out, n);

@@ -47,7 +47,7 @@ def test_calls_before_init_fail_cleanly():
     assert lib.u1_plaquette(C.byref(g), 0, None, out) == 3
     assert b"u1_init" in lib.u1_last_error()
     with pytest.raises(u1.U1Error):
-        u1.call("u1_update", C.byref(g), 0, None, 1, 3, C.byref(u1.BASELINE_METRO), 1, 0, 0, 0, 0, None)
+        u1.call("u1_update", C.byref(g), 0, None, 1, 3, C.byref(u1.BENCH_METRO), 1, 0, 0, 0, 0, None)
 
 
 def test_tune_cache_roundtrip(tmp_path):

@@ -1,15 +1,15 @@
-"""The BASELINE configurations other than the 48^4 headline, as GPU tests.
+"""Benchmark workloads other than the 48^4 headline, as GPU tests.
 
-C1  8^4 FP64 parity over 200 sweeps              -> test_gpu_parity.py
-C3  24^3 x 48, xi mapping beta_s = 1, beta_t = 2, stochastic 100-hit multihit
-    Polyakov correlator R = 1..12, 2000 sweeps measured every 10, FP32 links,
-    FP64 accumulation; FP64 oracle parity on fixed configurations
-C4  24^4 multilevel, 6 slabs of 4, n4 = n2 = 50, k4 = k2 = 1, 1 Metropolis +
-    3 OR per sublattice sweep, R = 1..10; oracle parity at reduced level counts
-C5  64 replicas of uneven shape, beta ~ U[0.9, 1.1], hot/cold mix, one graph
-    per sweep; per-replica bit parity (FP64) and <P> agreement (FP32 vs FP64)
+8^4 FP64 parity over 200 sweeps                      -> test_gpu_parity.py
+multihit, 24^3 x 48, xi mapping beta_s = 1, beta_t = 2: stochastic 100-hit
+    multihit Polyakov correlator R = 1..12, 2000 sweeps measured every 10,
+    FP32 links, FP64 accumulation; FP64 oracle parity on fixed configurations
+multilevel, 24^4, 6 slabs of 4, n4 = n2 = 50, k4 = k2 = 1, 1 Metropolis + 3 OR
+    per sublattice sweep, R = 1..10; oracle parity at reduced level counts
+ensemble, 64 replicas of uneven shape, beta ~ U[0.9, 1.1], hot/cold mix, one
+    graph per sweep; per-replica bit parity (FP64) and <P> agreement (FP32 vs FP64)
 
-Each test prints its timing and results (collected in DESIGN.md).
+Each test prints its timing and results.
 """
 import math
 import sys
@@ -38,23 +38,23 @@ def jackknife(x, nbins=20):
     return x[:n].mean(axis=0), np.sqrt((nbins - 1) / nbins * ((jk - jk.mean(axis=0)) ** 2).sum(axis=0))
 
 
-# C3: "xi = 2 (beta_s = 1, beta_t = beta_s * xi)": the reference weights
+# Multihit workload: "xi = 2 (beta_s = 1, beta_t = beta_s * xi)": the reference weights
 # space-space staples by beta/xi and space-time by beta*xi (staple.h:75-76),
 # so beta_s = 1, beta_t = 2 means beta = sqrt(2), xi = sqrt(2).
-C3_BETA = math.sqrt(2.0)
-C3_XI = math.sqrt(2.0)
+MH_BETA = math.sqrt(2.0)
+MH_XI = math.sqrt(2.0)
 
 
-def test_c3_multihit_fp64_parity_fixed_config(u1, orc):
+def test_multihit_24x3x48_fp64_parity_fixed_config(u1, orc):
     dims = (24, 24, 24, 48)
-    g = u1.geom(dims, C3_BETA, C3_XI)
-    og = orc.geom(dims, C3_BETA, C3_XI)
+    g = u1.geom(dims, MH_BETA, MH_XI)
+    og = orc.geom(dims, MH_BETA, MH_XI)
     lat = u1.Lattice.cold(g, u1.F64)
     lat.hotstart(33)
     ref = orc.hotstart(og, 33)
-    say("C3 parity: 24^3x48 xi=sqrt2, 3 sweeps + stochastic multihit")
+    say("multihit parity: 24^3x48 xi=sqrt2, 3 sweeps + stochastic multihit")
     for s in range(3):
-        lat.update(1, 3, u1.BASELINE_METRO, 33, s)
+        lat.update(1, 3, u1.BENCH_METRO, 33, s)
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 33, s)
     assert np.array_equal(bits(lat.to_host()), bits(ref))
     mh = u1.metro(1, 100, 0.5)
@@ -71,18 +71,18 @@ def test_c3_multihit_fp64_parity_fixed_config(u1, orc):
     rel_a = np.max(np.abs(a32 - a64)) / np.max(np.abs(a64))
     c32 = lat.astype(u1.F32).poly2(2, mh, seed=33, c1=7)
     rel_s = np.max(np.abs(c32 - c_orc)) / np.max(np.abs(c_orc))
-    say(f"C3 fixed config FP32 vs FP64: analytic multihit correlator max rel diff {rel_a:.2e}; "
+    say(f"multihit 24^3x48 fixed config FP32 vs FP64: analytic multihit correlator max rel diff {rel_a:.2e}; "
         f"stochastic (100 hits) {rel_s:.2e}")
     assert rel_a < 1e-5
     assert rel_s < 1e-2
 
 
-def test_c3_run_fp32_vs_fp64(u1):
+def test_multihit_24x3x48_fp32_vs_fp64_chains(u1):
     """2000 sweeps per chain, stochastic 100-hit multihit correlator every 10
     sweeps (the first 500 sweeps are thermalisation); 6 independent chains per
     precision, errors from the chain-to-chain spread."""
     dims = (24, 24, 24, 48)
-    g = u1.geom(dims, C3_BETA, C3_XI)
+    g = u1.geom(dims, MH_BETA, MH_XI)
     mh = u1.metro(1, 100, 0.5)
     nchain = 6
     res = {}
@@ -95,7 +95,7 @@ def test_c3_run_fp32_vs_fp64(u1):
             lat.hotstart(seed)
             series = []
             for s in range(2000):
-                lat.update(1, 3, u1.BASELINE_METRO, seed, s)
+                lat.update(1, 3, u1.BENCH_METRO, seed, s)
                 if s % 10 == 9 and s >= 500:
                     series.append(lat.poly2(2, mh, seed=seed, c1=s)[:12].real)
             means.append(np.mean(series, axis=0))
@@ -103,34 +103,34 @@ def test_c3_run_fp32_vs_fp64(u1):
         el = time.perf_counter() - t0
         means = np.array(means)
         res[name] = (means.mean(axis=0), means.std(axis=0, ddof=1) / math.sqrt(nchain))
-        say(f"C3 {name}: {nchain} chains x 2000 sweeps + 150 measurements each in {el:.2f} s; "
+        say(f"multihit 24^3x48 {name}: {nchain} chains x 2000 sweeps + 150 measurements each in {el:.2f} s; "
             f"C(R=1..4) = {np.array2string(res[name][0][:4], precision=4)} +- "
             f"{np.array2string(res[name][1][:4], precision=1)}")
     d = (res["fp32"][0] - res["fp64"][0]) / np.hypot(res["fp32"][1], res["fp64"][1])
-    say(f"C3 FP32 - FP64 in combined sigma, R=1..12: {np.array2string(d, precision=2)}")
+    say(f"multihit 24^3x48 FP32 - FP64 in combined sigma, R=1..12: {np.array2string(d, precision=2)}")
     assert np.all(np.abs(d[:6]) < 4.0)
 
 
-C4 = dict(n4=50, k4=1, n2=50, k2=1, metrop=1, ovrn=3)
+ML_PARAMS = dict(n4=50, k4=1, n2=50, k2=1, metrop=1, ovrn=3)
 
 
-def test_c4_multilevel_parity_reduced(u1, orc):
+def test_multilevel_24x4_parity_reduced(u1, orc):
     dims = (24, 24, 24, 24)
     g = u1.geom(dims, 1.0)
     og = orc.geom(dims, 1.0)
     lat = u1.Lattice.cold(g, u1.F64)
     lat.hotstart(44)
     ref = orc.hotstart(og, 44)
-    say("C4 parity: 24^4, 2 sweeps + multilevel n4=2 k4=1 n2=2 k2=1")
+    say("multilevel parity: 24^4, 2 sweeps + multilevel n4=2 k4=1 n2=2 k2=1")
     for s in range(2):
-        lat.update(1, 3, u1.BASELINE_METRO, 44, s)
+        lat.update(1, 3, u1.BENCH_METRO, 44, s)
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 44, s)
-    r_gpu = lat.multilevel(2, 1, 2, 1, 1, 3, u1.BASELINE_METRO, 44, call_id=1)
+    r_gpu = lat.multilevel(2, 1, 2, 1, 1, 3, u1.BENCH_METRO, 44, call_id=1)
     r_orc = orc.multilevel(og, ref, 2, 1, 2, 1, 1, 3, orc.metro(1, 4, 0.5), 44, call_id=1)
     assert np.allclose(r_gpu, r_orc, rtol=1e-10, atol=1e-20)
 
 
-def test_c4_multilevel_full(u1):
+def test_multilevel_24x4_full(u1):
     dims = (24, 24, 24, 24)
     g = u1.geom(dims, 1.0)
     out = {}
@@ -138,14 +138,15 @@ def test_c4_multilevel_full(u1):
         lat = u1.Lattice.cold(g, prec)
         lat.hotstart(404)
         for s in range(300):
-            lat.update(1, 3, u1.BASELINE_METRO, 404, s)
+            lat.update(1, 3, u1.BENCH_METRO, 404, s)
         u1.sync()
         t0 = time.perf_counter()
-        res = lat.multilevel(C4["n4"], C4["k4"], C4["n2"], C4["k2"], C4["metrop"], C4["ovrn"], u1.BASELINE_METRO,
+        res = lat.multilevel(ML_PARAMS["n4"], ML_PARAMS["k4"], ML_PARAMS["n2"], ML_PARAMS["k2"], ML_PARAMS["metrop"],
+                             ML_PARAMS["ovrn"], u1.BENCH_METRO,
                              404, call_id=0)
         el = time.perf_counter() - t0
         out[name] = res
-        say(f"C4 {name}: multilevel (n4=n2=50) in {el:.2f} s; C(R=1..10) = "
+        say(f"multilevel 24^4 {name}: (n4=n2=50) in {el:.2f} s; C(R=1..10) = "
             f"{np.array2string(res[:10].real, precision=3, floatmode='maxprec')}")
         assert np.all(np.isfinite(res))
         # the multilevel estimator of a real correlator: imaginary parts are noise
@@ -153,7 +154,7 @@ def test_c4_multilevel_full(u1):
     assert out["fp64"][0].real > 0 and out["fp32"][0].real > 0
 
 
-def c5_specs(seed=2024, n=64):
+def ensemble_specs(seed=2024, n=64):
     rng = np.random.default_rng(seed)
     shapes = [(8, 8, 8, 8), (12, 12, 12, 16), (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 32)]
     dims = [shapes[i] for i in rng.integers(0, len(shapes), n)]
@@ -163,11 +164,11 @@ def c5_specs(seed=2024, n=64):
     return dims, betas, hot, seeds
 
 
-def test_c5_batched_fp64_parity(u1, orc):
-    dims, betas, hot, seeds = c5_specs()
+def test_ensemble_64_uneven_fp64_parity(u1, orc):
+    dims, betas, hot, seeds = ensemble_specs()
     geoms = [u1.geom(d, b) for d, b in zip(dims, betas)]
-    e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BASELINE_METRO)
-    say("C5 parity: 64 replicas, 2 graph sweeps, FP64 vs oracle")
+    e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BENCH_METRO)
+    say("ensemble parity: 64 replicas, 2 graph sweeps, FP64 vs oracle")
     e.sweep(2, graph=True)
     p = e.plaquette()
     for r in range(len(dims)):
@@ -180,20 +181,20 @@ def test_c5_batched_fp64_parity(u1, orc):
     e.destroy()
 
 
-def test_c5_batched_fp32_vs_fp64(u1):
-    dims, betas, hot, seeds = c5_specs()
+def test_ensemble_64_uneven_fp32_vs_fp64(u1):
+    dims, betas, hot, seeds = ensemble_specs()
     geoms = [u1.geom(d, b) for d, b in zip(dims, betas)]
     n_links = sum(4 * int(np.prod(d)) for d in dims)
     series = {}
     for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
-        e = u1.Ensemble(geoms, seeds, hot, prec, 1, 3, u1.BASELINE_METRO)
+        e = u1.Ensemble(geoms, seeds, hot, prec, 1, 3, u1.BENCH_METRO)
         e.sweep(200, graph=True)  # thermalisation
         u1.sync()
         t0 = time.perf_counter()
         e.sweep(300, graph=True)
         u1.sync()
         el = time.perf_counter() - t0
-        say(f"C5 {name}: 64 replicas, {n_links} links, 300 graph sweeps in {el:.3f} s -> "
+        say(f"ensemble {name}: 64 replicas, {n_links} links, 300 graph sweeps in {el:.3f} s -> "
             f"{4 * n_links * 300 / el:.3e} link updates/s")
         vals = []
         for s in range(200):
@@ -204,6 +205,6 @@ def test_c5_batched_fp32_vs_fp64(u1):
     m32, e32 = jackknife(series["fp32"], 20)
     m64, e64 = jackknife(series["fp64"], 20)
     z = (m32 - m64) / np.hypot(e32, e64)
-    say(f"C5 per-replica <P> FP32 - FP64 in combined sigma: mean |z| {np.mean(np.abs(z)):.2f}, max |z| "
+    say(f"ensemble per-replica <P> FP32 - FP64 in combined sigma: mean |z| {np.mean(np.abs(z)):.2f}, max |z| "
         f"{np.max(np.abs(z)):.2f}, fraction within 1 sigma {np.mean(np.abs(z) < 1):.2f}")
     assert np.max(np.abs(z)) < 5.0

@@ -98,7 +98,7 @@ def test_fp64_sweep_parity_8x4(u1, orc, start, mode):
     theta + U[-0.5,0.5) with 4 hits, or the reference's independent proposal."""
     dims = (8, 8, 8, 8)
     g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, start == "hot", seed=1234)
-    m = u1.BASELINE_METRO if mode == "baseline" else u1.REFERENCE_METRO
+    m = u1.BENCH_METRO if mode == "baseline" else u1.REFERENCE_METRO
     om = orc.metro(m.proposal, m.nhit, m.step)
     say(f"fp64 parity 8^4 start={start} mode={mode}")
     assert np.array_equal(bits(lat.to_host()), bits(ref))
@@ -126,7 +126,7 @@ def test_fp64_sweep_parity_8x4(u1, orc, start, mode):
 def test_fp64_parity_geometries(u1, orc, dims, beta, aniso, metrop, ovrn, nsw):
     g, og, lat, ref = make_pair(u1, orc, dims, beta, aniso, True, seed=77)
     say(f"fp64 parity {dims} beta={beta} xi={aniso}")
-    m = u1.BASELINE_METRO
+    m = u1.BENCH_METRO
     om = orc.metro(1, 4, 0.5)
     for s in range(nsw):
         assert lat.update(metrop, ovrn, m, 77, s, count_accepts=True) == \
@@ -168,7 +168,7 @@ def test_fused_plaquette_fp64(u1, orc, dims, aniso, metrop, ovrn):
     result within 1e-13 of the oracle's."""
     g, og, lat, ref = make_pair(u1, orc, dims, 1.0, aniso, True, seed=9)
     say(f"fused plaquette fp64 {dims} xi={aniso} metrop={metrop} ovrn={ovrn}")
-    m = u1.BASELINE_METRO
+    m = u1.BENCH_METRO
     for s in range(3):
         p = lat.update_plaquette(metrop, ovrn, m, 9, s)
         orc.update(og, ref, metrop, ovrn, orc.metro(1, 4, 0.5), 9, s)
@@ -186,12 +186,25 @@ def test_fused_plaquette_fp32():
         lat = u1.Lattice.cold(g, u1.F32)
         lat.hotstart(21)
         for s in range(3):
-            p = lat.update_plaquette(1, 3, u1.BASELINE_METRO, 21, s)
+            p = lat.update_plaquette(1, 3, u1.BENCH_METRO, 21, s)
             q = lat.plaquette()
             for k in (0, 2):
                 assert abs(p[k] - q[k]) <= 2e-6 * abs(q[k]), (dims, s, p, q)
             for k in (1, 3):
                 assert abs(p[k] - q[k]) <= 2e-6, (dims, s, p, q)
+
+
+def test_stream_reference(u1):
+    """u1_selftest_stream moves 36 n bytes at a plausible HBM rate and checks its arguments."""
+    n = 1 << 21
+    for pattern in (0, 1, 2):
+        r = u1.stream_rate(pattern, n, 5)
+        say(f"stream pattern {pattern}: {r:.0f} GB/s")
+        assert 500.0 < r < 9000.0
+    with pytest.raises(u1.U1Error):
+        u1.stream_rate(2, n + 4, 1)
+    with pytest.raises(u1.U1Error):
+        u1.stream_rate(3, n, 1)
 
 
 def test_phase_write_set_gpu(u1, orc):
@@ -203,7 +216,7 @@ def test_phase_write_set_gpu(u1, orc):
         for parity in (0, 1):
             for mu in range(4):
                 before = lat.to_host()
-                lat.phase(kind, parity, mu, u1.BASELINE_METRO, 5, sweep=kind)
+                lat.phase(kind, parity, mu, u1.BENCH_METRO, 5, sweep=kind)
                 after = lat.to_host()
                 ch = np.nonzero(bits(after) != bits(before))[0]
                 lo = mu * V + parity * (V // 2)
@@ -215,7 +228,7 @@ def test_masked_multilevel_update_parity(u1, orc):
     g, og, lat, ref = make_pair(u1, orc, dims, seed=9)
     for ml, stream in ((4, 3), (2, 4)):
         for s in range(3):
-            lat.update(1, 3, u1.BASELINE_METRO, 9, s, stream=stream, c3=5, mask_ml=ml)
+            lat.update(1, 3, u1.BENCH_METRO, 9, s, stream=stream, c3=5, mask_ml=ml)
             orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 9, s, stream=stream, c3=5, mask_ml=ml)
     assert np.array_equal(bits(lat.to_host()), bits(ref))
 
@@ -227,7 +240,7 @@ def thermal(u1, orc):
     dims = (8, 8, 8, 8)
     g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=4321)
     for s in range(40):
-        lat.update(1, 3, u1.BASELINE_METRO, 4321, s)
+        lat.update(1, 3, u1.BENCH_METRO, 4321, s)
     for s in range(40):
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 4321, s)
     assert np.array_equal(bits(lat.to_host()), bits(ref))
@@ -259,11 +272,11 @@ def test_multilevel_fp64(u1, orc, mh_mode):
     dims = (4, 4, 4, 8)
     g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=19)
     for s in range(5):
-        lat.update(1, 3, u1.BASELINE_METRO, 19, s)
+        lat.update(1, 3, u1.BENCH_METRO, 19, s)
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 19, s)
     say(f"multilevel fp64 mh_mode={mh_mode}")
     mh = u1.metro(1, 20, 0.5)
-    r_gpu, f_gpu = lat.multilevel(2, 1, 3, 1, 1, 2, u1.BASELINE_METRO, 19, call_id=2, mh_mode=mh_mode, mh=mh,
+    r_gpu, f_gpu = lat.multilevel(2, 1, 3, 1, 1, 2, u1.BENCH_METRO, 19, call_id=2, mh_mode=mh_mode, mh=mh,
                                   field=True)
     r_orc, f_orc = orc.multilevel(og, ref, 2, 1, 3, 1, 1, 2, orc.metro(1, 4, 0.5), 19, call_id=2, mh_mode=mh_mode,
                                   mh=orc.metro(1, 20, 0.5), field=True)
@@ -277,7 +290,7 @@ def test_multilevel_errors(u1):
     g = u1.geom((4, 4, 4, 6))
     lat = u1.Lattice.cold(g)
     with pytest.raises(u1.U1Error):
-        lat.multilevel(1, 1, 1, 1, 1, 1, u1.BASELINE_METRO, 1)
+        lat.multilevel(1, 1, 1, 1, 1, 1, u1.BENCH_METRO, 1)
     with pytest.raises(u1.U1Error):
         u1.Lattice.cold(u1.geom((4, 4, 3, 4)))  # allocation is fine...
         bad = u1.Lattice(u1.geom((4, 4, 3, 4)), 0, u1.DeviceArray(4 * 48))
@@ -325,8 +338,8 @@ def test_fp32_overrelaxation_tracks_fp64(u1, orc):
     start = orc.hotstart(og, 3).astype(np.float32).astype(np.float64)
     a64 = u1.Lattice.from_host(g, start, u1.F64)
     a32 = u1.Lattice.from_host(g, start, u1.F32)
-    a64.update(0, 1, u1.BASELINE_METRO, 3, 0)
-    a32.update(0, 1, u1.BASELINE_METRO, 3, 0)
+    a64.update(0, 1, u1.BENCH_METRO, 3, 0)
+    a32.update(0, 1, u1.BENCH_METRO, 3, 0)
     d = np.angle(np.exp(1j * (a64.to_host() - a32.to_host().astype(np.float64))))
     assert np.median(np.abs(d)) < 1e-5
 
@@ -341,7 +354,7 @@ def test_fp32_chain_statistics_match_fp64(u1):
         lat.hotstart(99)
         v = []
         for s in range(400):
-            lat.update(1, 3, u1.BASELINE_METRO, 99, s)
+            lat.update(1, 3, u1.BENCH_METRO, 99, s)
             if s >= 100:
                 v.append(u1.mean_plaquette(lat.plaquette()))
         vals[prec] = np.array(v)
@@ -359,7 +372,7 @@ def test_ensemble_fp64_matches_oracle(u1, orc, graph):
     geoms = [u1.geom(d, b, x) for d, b, x, _ in specs]
     seeds = [101 + i for i in range(len(specs))]
     hot = [h for *_, h in specs]
-    e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BASELINE_METRO)
+    e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BENCH_METRO)
     say(f"ensemble fp64 graph={graph}")
     e.sweep(6, graph=graph)
     p = e.plaquette()
@@ -375,7 +388,7 @@ def test_ensemble_fp64_matches_oracle(u1, orc, graph):
 
 def test_ensemble_fp32_runs(u1):
     geoms = [u1.geom((8, 8, 8, 8), 0.95), u1.geom((12, 12, 12, 16), 1.05)]
-    e = u1.Ensemble(geoms, [1, 2], [1, 0], u1.F32, 1, 3, u1.BASELINE_METRO)
+    e = u1.Ensemble(geoms, [1, 2], [1, 0], u1.F32, 1, 3, u1.BENCH_METRO)
     e.sweep(20, graph=True)
     p = e.plaquette()
     assert np.all(np.isfinite(p))
@@ -400,16 +413,16 @@ def test_checkpoint_resume_is_exact(u1, tmp_path, prec):
     a = u1.Lattice.cold(g, prec)
     a.hotstart(5)
     for s in range(4):
-        a.update(1, 3, u1.BASELINE_METRO, 5, s)
+        a.update(1, 3, u1.BENCH_METRO, 5, s)
     path = str(tmp_path / "cfg.u1")
     a.save(path, 5, 4)
     for s in range(4, 8):
-        a.update(1, 3, u1.BASELINE_METRO, 5, s)
+        a.update(1, 3, u1.BENCH_METRO, 5, s)
     b = u1.Lattice.cold(g, prec)
     seed, sweep = b.load(path)
     assert (seed, sweep) == (5, 4)
     for s in range(sweep, 8):
-        b.update(1, 3, u1.BASELINE_METRO, seed, s)
+        b.update(1, 3, u1.BENCH_METRO, seed, s)
     assert np.array_equal(a.to_host().view(np.uint8), b.to_host().view(np.uint8))
     # the file holds FP64 angles in lexicographic order
     raw = np.fromfile(path, dtype=np.float64, offset=64)

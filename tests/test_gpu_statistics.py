@@ -22,7 +22,7 @@ def say(*a):
 def replica_means(u1, prec, beta, nrep, dims, ntherm, nmeas, seed0):
     geoms = [u1.geom(dims, beta) for _ in range(nrep)]
     seeds = [seed0 + 7919 * i for i in range(nrep)]
-    e = u1.Ensemble(geoms, seeds, [1] * nrep, prec, 1, 3, u1.BASELINE_METRO)
+    e = u1.Ensemble(geoms, seeds, [1] * nrep, prec, 1, 3, u1.BENCH_METRO)
     e.sweep(ntherm, graph=True)
     acc = np.zeros(nrep)
     for _ in range(nmeas):
