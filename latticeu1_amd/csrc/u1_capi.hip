@@ -262,9 +262,12 @@ struct GroupEvents {
 // meas_dst (device, 4 doubles, nullable): the plaquette of the updated
 // lattice, in u1_plaquette's normalisation, from the measuring variants of
 // phases (parity 1, mu = 1, 2, 3) of the sweep's last pass.
+// sweep_dev (nullable): the Philox sweep index is *sweep_dev * sweep_mul +
+// sweep, read by the kernels (for graph capture; phase_cfg must have run).
 int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, const u1_metro* m, uint64_t seed,
                uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
-               const GroupEvents* gev = nullptr, double* meas_dst = nullptr) {
+               const GroupEvents* gev = nullptr, double* meas_dst = nullptr, const uint32_t* sweep_dev = nullptr,
+               uint32_t sweep_mul = 0) {
   PhaseArgs a{};
   a.single = make_rep(g, seed, c3);
   a.reps = nullptr;
@@ -275,7 +278,8 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   a.step = m->step;
   a.stream = stream;
   a.sweep = sweep;
-  a.sweep_dev = nullptr;
+  a.sweep_dev = sweep_dev;
+  a.sweep_mul = sweep_mul;
   a.accepts = acc_dev;
   Cfg cm = kDefaultCfg, co = kDefaultCfg;
   if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
@@ -412,6 +416,7 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
   a.stream = u1r::STREAM_UPDATE;
   a.sweep = 0;
   a.sweep_dev = e->sweep_dev;
+  a.sweep_mul = 1;
   a.accepts = nullptr;
   a.xcd = (uint32_t)e->cfg.xcd;
   for (int p = 0; p < e->metrop; ++p)
@@ -760,13 +765,19 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   const int64_t nl2 = r.N[3] / 2, nl4 = r.N[3] / 4;
   const size_t latb = 4 * (size_t)r.V * elem_size(prec);
   void *wl = nullptr, *tl = nullptr, *l2 = nullptr, *l4 = nullptr;
+  uint32_t* ctr = nullptr;  // level-2 step counter i * n2 + k, advanced by the graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
   int rc = U1_OK;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(G.st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (wl) (void)hipFree(wl);
     if (tl) (void)hipFree(tl);
     if (l2) (void)hipFree(l2);
     if (l4) (void)hipFree(l4);
+    if (ctr) (void)hipFree(ctr);
   };
 #define MLCHK(expr)                                                                          \
   do {                                                                                       \
@@ -782,25 +793,56 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   MLCHK(hipMalloc(&l4, 2 * (size_t)sz * nl4 * sizeof(double)));
   MLCHK(hipMemcpyAsync(wl, lat, latb, hipMemcpyDeviceToDevice, G.st));
   MLCHK(u1d::launch_fill_zero(l4, 2 * sz * nl4 * (int64_t)sizeof(double), G.st));
+  MLCHK(hipMalloc((void**)&ctr, sizeof(uint32_t)));
+  MLCHK(u1d::launch_fill_zero(ctr, sizeof(uint32_t), G.st));
   const double l2norm = 1.0 / (double)n2, l4norm = 1.0 / (double)n4;
   const int mp = mh ? mh->proposal : 0, mn = mh ? mh->nhit : 0;
   const double ms = mh ? mh->step : 0.0;
+  {  // launch shapes are settled before capture (the tuner allocates and synchronises)
+    PhaseArgs a{};
+    a.single = r;
+    a.nhit = m->nhit;
+    a.proposal = m->proposal;
+    a.step = m->step;
+    Cfg c;
+    if (metrop > 0) rc = phase_cfg(prec, u1d::KIND_METROPOLIS, g, wl, a, &c);
+    if (rc == U1_OK && ovrn > 0) rc = phase_cfg(prec, u1d::KIND_OVERRELAX, g, wl, a, &c);
+    if (rc != U1_OK) {
+      cleanup();
+      return rc;
+    }
+  }
+  // One level-2 step -- k2 masked sweeps (Philox sweep index ctr * k2 + l),
+  // the temporal-link field (c1 = ctr), the T2 accumulation and ctr += 1 --
+  // captured once and replayed n4 * n2 times (multilevel.cu:680-700 order;
+  // sweep and c1 indices equal to ((i n2 + k) k2 + l) and (i n2 + k)).
+  MLCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
+  for (int l = 0; l < k2 && rc == U1_OK; ++l)
+    rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)l, call_id, 2, nullptr, nullptr,
+                    nullptr, ctr, (uint32_t)k2);
+  if (rc == U1_OK) {
+    const hipError_t e1 = u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT, 0,
+                                                     call_id, (double*)tl, 256, G.st, ctr);
+    const hipError_t e2 = e1 == hipSuccess ? u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st) : e1;
+    const hipError_t e3 = e2 == hipSuccess ? u1d::launch_increment(ctr, G.st) : e2;
+    if (e3 != hipSuccess) rc = fail(U1_ERR_HIP, std::string("multilevel capture: ") + hipGetErrorString(e3));
+  }
+  {
+    const hipError_t ce = hipStreamEndCapture(G.st, &graph);
+    if (rc != U1_OK) {
+      cleanup();
+      return rc;
+    }
+    MLCHK(ce);
+  }
+  MLCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   for (int i = 0; i < n4 && rc == U1_OK; ++i) {
     for (int j = 0; j < k4 && rc == U1_OK; ++j)
       rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L4, (uint32_t)(i * k4 + j), call_id, 4,
                       nullptr);
     if (rc != U1_OK) break;
     MLCHK(u1d::launch_fill_zero(l2, 2 * sz * nl2 * (int64_t)sizeof(double), G.st));
-    for (int k = 0; k < n2 && rc == U1_OK; ++k) {
-      for (int l = 0; l < k2 && rc == U1_OK; ++l)
-        rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)((i * n2 + k) * k2 + l),
-                        call_id, 2, nullptr);
-      if (rc != U1_OK) break;
-      MLCHK(u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT,
-                                       (uint32_t)(i * n2 + k), call_id, (double*)tl, 256, G.st));
-      MLCHK(u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st));
-    }
-    if (rc != U1_OK) break;
+    for (int k = 0; k < n2; ++k) MLCHK(hipGraphLaunch(exec, G.st));
     MLCHK(u1d::launch_ml_l4((const double*)l2, (double*)l4, r, rmax, l2norm, 256, G.st));
   }
   if (rc == U1_OK) {

@@ -21,7 +21,8 @@ struct PhaseArgs {
   double step;
   uint32_t stream, pass, sweep;
   uint32_t xcd;                 // 1: XCD-aware workgroup order (xcd_remap)
-  const uint32_t* sweep_dev;    // if set, the sweep index is read here (graph replays)
+  const uint32_t* sweep_dev;    // if set, sweep index = *sweep_dev * sweep_mul + sweep (graph replays)
+  uint32_t sweep_mul;
   unsigned long long* accepts;  // if set, accepted hits are added here
   double* meas;                 // if set (mu >= 1, single lattice): plaquette rows, 4 doubles per workgroup
 };
@@ -55,9 +56,10 @@ hipError_t launch_finish_sums(const double* partials, int nblk, int nv, double s
 hipError_t launch_poly_volume(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
                               double step, uint32_t stream, uint32_t c1, uint32_t c3, double* P, int block,
                               hipStream_t st);
+// c1_dev (nullable): Philox counter word c1 read from device memory (graph replays)
 hipError_t launch_temporal_links(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
                                  double step, uint32_t stream, uint32_t c1, uint32_t c3, double* out, int block,
-                                 hipStream_t st);
+                                 hipStream_t st, const uint32_t* c1_dev = nullptr);
 // P(x) = prod_t T(x, t) from a temporal-link field (2*V doubles, lexicographic 4D)
 hipError_t launch_tprod(const double* tl, const RepDesc& r, double* P, hipStream_t st);
 hipError_t launch_pp(const double* P, const RepDesc& r, int rmax, double* partials /*2*rmax*nblk*/, int nblk,

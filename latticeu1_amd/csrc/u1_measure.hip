@@ -259,9 +259,11 @@ __global__ __launch_bounds__(256) void k_poly_volume(const Real* __restrict__ la
 template <typename Real>
 __global__ __launch_bounds__(256) void k_temporal_links(const Real* __restrict__ lat, RepDesc r, int mode,
                                                         int proposal, int nhit, double step, uint32_t stream,
-                                                        uint32_t c1, uint32_t c3, double* __restrict__ out) {
+                                                        uint32_t c1, uint32_t c3, double* __restrict__ out,
+                                                        const uint32_t* c1_dev) {
   const int32_t idx = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (idx >= r.V) return;
+  if (c1_dev) c1 = *c1_dev;
   const int32_t parity = idx >= r.HV ? 1 : 0;
   const int32_t id = idx - parity * r.HV;
   const C64 v = temporal_value<Real>(lat, r, id, parity, mode, proposal, nhit, step, stream, c1, c3);
@@ -562,14 +564,14 @@ hipError_t launch_tprod(const double* tl, const RepDesc& r, double* P, hipStream
 
 hipError_t launch_temporal_links(int prec, const void* lat, const RepDesc& r, int mode, int proposal, int nhit,
                                  double step, uint32_t stream, uint32_t c1, uint32_t c3, double* out, int block,
-                                 hipStream_t st) {
+                                 hipStream_t st, const uint32_t* c1_dev) {
   const int nb = nblocks(r.V, block);
   if (prec == PREC_F64)
     k_temporal_links<double><<<nb, block, 0, st>>>((const double*)lat, r, mode, proposal, nhit, step, stream, c1, c3,
-                                                   out);
+                                                   out, c1_dev);
   else
     k_temporal_links<float><<<nb, block, 0, st>>>((const float*)lat, r, mode, proposal, nhit, step, stream, c1, c3,
-                                                  out);
+                                                  out, c1_dev);
   return hipGetLastError();
 }
 

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
       m[k].acc = 0;
       l[k] = same[k] + (uint32_t)MU * (uint32_t)r.V;
     }
-    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev * a.sweep_mul + a.sweep : a.sweep;
     const Real beta = (Real)r.beta;
     for (int h = 0; h < a.nhit; h += 2) {
       const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));

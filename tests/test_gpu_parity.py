@@ -267,19 +267,25 @@ def test_polyakov_and_correlators_fp64(u1, orc, thermal):
         assert np.allclose(t_gpu, t_orc, rtol=1e-12, atol=1e-15)
 
 
-@pytest.mark.parametrize("mh_mode", [1, 2])
-def test_multilevel_fp64(u1, orc, mh_mode):
+@pytest.mark.parametrize("mh_mode,n4,k4,n2,k2", [
+    (1, 2, 1, 3, 1),
+    (2, 2, 1, 3, 1),
+    (0, 3, 0, 4, 2),   # no level-4 updates, two level-2 sweeps per step (graph replays 12 times)
+])
+def test_multilevel_fp64(u1, orc, mh_mode, n4, k4, n2, k2):
+    """The level-2 step runs as one captured graph replayed n4 * n2 times with
+    device-side Philox counters; every replay must match the oracle's eager loop."""
     dims = (4, 4, 4, 8)
     g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=19)
     for s in range(5):
         lat.update(1, 3, u1.BENCH_METRO, 19, s)
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 19, s)
-    say(f"multilevel fp64 mh_mode={mh_mode}")
+    say(f"multilevel fp64 mh_mode={mh_mode} n4={n4} k4={k4} n2={n2} k2={k2}")
     mh = u1.metro(1, 20, 0.5)
-    r_gpu, f_gpu = lat.multilevel(2, 1, 3, 1, 1, 2, u1.BENCH_METRO, 19, call_id=2, mh_mode=mh_mode, mh=mh,
+    r_gpu, f_gpu = lat.multilevel(n4, k4, n2, k2, 1, 2, u1.BENCH_METRO, 19, call_id=2, mh_mode=mh_mode, mh=mh,
                                   field=True)
-    r_orc, f_orc = orc.multilevel(og, ref, 2, 1, 3, 1, 1, 2, orc.metro(1, 4, 0.5), 19, call_id=2, mh_mode=mh_mode,
-                                  mh=orc.metro(1, 20, 0.5), field=True)
+    r_orc, f_orc = orc.multilevel(og, ref, n4, k4, n2, k2, 1, 2, orc.metro(1, 4, 0.5), 19, call_id=2,
+                                  mh_mode=mh_mode, mh=orc.metro(1, 20, 0.5), field=True)
     assert np.allclose(r_gpu, r_orc, rtol=1e-11, atol=1e-16)
     assert np.allclose(f_gpu, f_orc, rtol=1e-11, atol=1e-16)
     # lattice passed in is not modified (multilevel works on a copy)
