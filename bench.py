@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20241)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="print every autotuner candidate (stderr)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--plaq", choices=["fused", "separate"], default="fused",
                     help="plaquette every sweep: summed inside the last overrelaxation pass (fused) or by a "
@@ -184,6 +185,8 @@ def main():
 
     u1.init(0)
     dev = u1.device_name()
+    if args.verbose:
+        u1.set_verbosity(2)
     if args.tunecache and os.path.exists(args.tunecache):
         u1.tune_load(args.tunecache)
 

@@ -190,6 +190,10 @@ int u1_ensemble_destroy(u1_ensemble* e);
 /* Tuning picks the workgroup size per (kernel, geometry, precision) by timing
  * candidates on a scratch copy (results are independent of the choice). */
 int u1_tune_enable(int32_t on);
+/* Verbosity (parameters.h:33-37): 0 SILENT, 1 SUMMARIZE (the tuner reports
+ * its choice on stderr), 2 VERBOSE and 3 DEBUG_VERBOSE (every candidate's
+ * time).  Callable before u1_init. */
+int u1_set_verbosity(int32_t level);
 int u1_tune_load(const char* path);
 int u1_tune_save(const char* path);
 /* Number of tuned entries; if key/val are given, entry i as text. */

@@ -108,6 +108,7 @@ _SIGS = {
     "u1_ensemble_info": (_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int64)),
     "u1_ensemble_destroy": (_P,),
     "u1_tune_enable": (C.c_int32,),
+    "u1_set_verbosity": (C.c_int32,),
     "u1_tune_load": (C.c_char_p,),
     "u1_tune_save": (C.c_char_p,),
     "u1_tune_entries": (C.POINTER(C.c_int32),),
@@ -432,6 +433,11 @@ def selftest_philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
     call("u1_selftest_philox", ctr.ctypes.data_as(u32), key.ctypes.data_as(u32), out.ctypes.data_as(u32),
          ctr.shape[0])
     return out
+
+
+def set_verbosity(level: int) -> None:
+    """0 silent, 1 the tuner's choices, 2 every tuning candidate (stderr)."""
+    call("u1_set_verbosity", level)
 
 
 def tune_enable(on: bool) -> None:

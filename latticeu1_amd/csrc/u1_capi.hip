@@ -37,6 +37,7 @@ struct Ctx {
   unsigned long long* counter = nullptr;
   double* dres = nullptr;  // small device result buffer (64 doubles)
   bool tune = true;
+  int verbosity = 0;                      // 0 silent, 1 summarize, 2 verbose (tuner output on stderr)
   std::map<std::string, Cfg> tunecache;  // key -> launch shape
   int tune_loaded = 0;
 };
@@ -223,33 +224,45 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
   void* scratch = nullptr;
   HIPCHK(hipMalloc(&scratch, bytes));
   HIPCHK(hipMemcpyAsync(scratch, lat, bytes, hipMemcpyDeviceToDevice, G.st));
-  float best = 1e30f;
-  Cfg bestc = kDefaultCfg;
-  for (const Cfg c : candidates()) {
-    if (c.spt == 4 && prec != U1_F32) continue;
-    PhaseArgs a = proto;
-    a.accepts = nullptr;
-    a.xcd = (uint32_t)c.xcd;
-    set_blocks(a.single, c.block * c.spt);
-    float ms = 0;
-    int rc = time_phase(prec, kind, scratch, a, a.single.nblk, c, &ms);
-    if (rc != U1_OK) {
-      (void)hipFree(scratch);
-      return rc;
+  // Three rounds over all candidates, best time per candidate: a round-robin
+  // order keeps clock ramp-up and neighbour noise from favouring one shape.
+  std::vector<Cfg> cands;
+  for (const Cfg c : candidates())
+    if (!(c.spt == 4 && prec != U1_F32)) cands.push_back(c);
+  std::vector<float> tbest(cands.size(), 1e30f);
+  for (int round = 0; round < 3; ++round)
+    for (size_t i = 0; i < cands.size(); ++i) {
+      const Cfg c = cands[i];
+      PhaseArgs a = proto;
+      a.accepts = nullptr;
+      a.meas = nullptr;
+      a.xcd = (uint32_t)c.xcd;
+      set_blocks(a.single, c.block * c.spt);
+      float ms = 0;
+      int rc = time_phase(prec, kind, scratch, a, a.single.nblk, c, &ms);
+      if (rc != U1_OK) {
+        (void)hipFree(scratch);
+        return rc;
+      }
+      tbest[i] = std::min(tbest[i], ms);
     }
-    if (ms < best) {
-      best = ms;
-      bestc = c;
-    }
-  }
   HIPCHK(hipFree(scratch));
-  G.tunecache[key] = bestc;
-  *cfg = bestc;
+  size_t ib = 0;
+  for (size_t i = 1; i < cands.size(); ++i)
+    if (tbest[i] < tbest[ib]) ib = i;
+  if (G.verbosity >= 2)
+    for (size_t i = 0; i < cands.size(); ++i)
+      std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order: %.2f us per launch%s\n", key.c_str(),
+                   cands[i].block, cands[i].spt, cands[i].xcd ? "XCD" : "plain", 1e3 * tbest[i] / 8.0,
+                   i == ib ? "  <- chosen" : "");
+  else if (G.verbosity >= 1)
+    std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order: %.2f us per launch\n", key.c_str(),
+                 cands[ib].block, cands[ib].spt, cands[ib].xcd ? "XCD" : "plain", 1e3 * tbest[ib] / 8.0);
+  G.tunecache[key] = cands[ib];
+  *cfg = cands[ib];
   return U1_OK;
 }
 
-// Events bracketing the Metropolis and overrelaxation groups of one sweep
-// (for the mean launch duration of each kernel class).
 // Events around one sweep's launch groups: start | Metropolis | after_metro |
 // OR phases without measurement | before_meas | measuring phases (and the OR
 // phases between them) | after_or.  Without measurement before_meas is
@@ -1120,6 +1133,12 @@ int u1_ensemble_destroy(u1_ensemble* e) {
 // ---------------------------------------------------------------- autotuner
 int u1_tune_enable(int32_t on) {
   G.tune = on != 0;
+  return U1_OK;
+}
+
+int u1_set_verbosity(int32_t level) {
+  if (level < 0 || level > 3) return fail(U1_ERR_ARG, "verbosity must be in [0, 3]");
+  G.verbosity = level;
   return U1_OK;
 }
 

@@ -154,7 +154,10 @@ void setTuning(TuneMode t) {
   kerneltune = t;
   u1_tune_enable(t == TUNE_YES ? 1 : 0);
 }
-void setVerbosity(Verbosity v) { verbose = v; }
+void setVerbosity(Verbosity v) {
+  verbose = v;
+  (void)u1_set_verbosity((int32_t)v);  // tuner reporting follows the same level
+}
 
 void setProposal(ProposalKind kind, int nhit, double step) {
   PARAMS::proposal = kind;
