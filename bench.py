@@ -38,51 +38,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from latticeu1_amd.analysis import gamma_method, jackknife  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
-
-
-def jackknife(x, nbins=20):
-    x = np.asarray(x, dtype=np.float64)
-    n = len(x) // nbins * nbins
-    if n < nbins:
-        return float(np.mean(x)), float("nan")
-    b = x[:n].reshape(nbins, -1).mean(axis=1)
-    tot = b.sum()
-    jk = (tot - b) / (nbins - 1)
-    mean = jk.mean()
-    err = math.sqrt((nbins - 1) / nbins * np.sum((jk - mean) ** 2))
-    return float(x[:n].mean()), err
-
-
-def gamma_method(a, S=1.5):
-    """Wolff's Gamma method (hep-lat/0306017): mean, error with autocorrelations,
-    integrated autocorrelation time, window.  Same algorithm as include/u1/actime.h."""
-    a = np.asarray(a, dtype=np.float64)
-    N = len(a)
-    d = a - a.mean()
-    tmax = N // 2
-    G = [float(np.dot(d, d) / N)]
-    sum_rho, W, found = 0.0, 0, False
-    t = 1
-    while t <= tmax:
-        G.append(float(np.dot(d[:N - t], d[t:]) / (N - t)))
-        if not found:
-            if G[0] <= 0:
-                break
-            sum_rho += G[t] / G[0]
-            tau = 1e-300 if sum_rho <= 0 else S / math.log((sum_rho + 1) / sum_rho)
-            if math.exp(-t / tau) - tau / math.sqrt(t * N) < 0:
-                W, found = t, True
-                tmax = min(tmax, 2 * t)
-        t += 1
-    if not found:
-        W = len(G) - 1
-    if G[0] <= 0:
-        return float(a.mean()), 0.0, 0.5, W
-    C = G[0] + 2 * sum(G[1:W + 1])
-    G = [g + C / N for g in G]
-    C = G[0] + 2 * sum(G[1:W + 1])
-    return float(a.mean()), math.sqrt(C / N), C / (2 * G[0]), W
 
 
 def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):

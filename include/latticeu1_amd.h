@@ -178,6 +178,12 @@ typedef struct u1_ensemble u1_ensemble;
 int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, const uint64_t* seeds,
                        const int32_t* hot, int prec, int32_t metrop, int32_t ovrn, const u1_metro* m);
 int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph);
+/* nsweeps graph sweeps of all replicas, each with the plaquette of every
+ * replica summed inside its last pass (as u1_update_plaquette); series
+ * (host, nsweeps * nrep * 4 doubles) receives them sweep-major in
+ * u1_plaquette's form.  No counterpart in the reference, which holds one
+ * lattice per process and measures with Plaquette (include/plaquette.h:12). */
+int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series);
 /* out[4*nrep]: per-replica plaquette as u1_plaquette. */
 int u1_ensemble_plaquette(u1_ensemble* e, double* out);
 /* Replica r angles as FP64 host array (4V, EO layout); write sets them. */

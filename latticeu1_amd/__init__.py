@@ -102,6 +102,7 @@ _SIGS = {
     "u1_ensemble_create": (C.POINTER(_P), C.c_int32, C.POINTER(Geom), C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
                            C.c_int, C.c_int32, C.c_int32, C.POINTER(Metro)),
     "u1_ensemble_sweep": (_P, C.c_int32, C.c_int32),
+    "u1_ensemble_sweep_plaquette": (_P, C.c_int32, C.POINTER(C.c_double)),
     "u1_ensemble_plaquette": (_P, C.POINTER(C.c_double)),
     "u1_ensemble_read": (_P, C.c_int32, C.POINTER(C.c_double)),
     "u1_ensemble_write": (_P, C.c_int32, C.POINTER(C.c_double)),
@@ -380,6 +381,13 @@ class Ensemble:
 
     def sweep(self, nsweeps: int = 1, graph: bool = True) -> None:
         call("u1_ensemble_sweep", self.ptr, nsweeps, 1 if graph else 0)
+
+    def sweep_plaquette(self, nsweeps: int) -> np.ndarray:
+        """nsweeps graph sweeps with each replica's plaquette summed inside the
+        sweep; returns (nsweeps, nrep, 4) in plaquette() form."""
+        out = np.zeros(4 * self.n * nsweeps)
+        call("u1_ensemble_sweep_plaquette", self.ptr, nsweeps, _dp(out))
+        return out.reshape(nsweeps, self.n, 4)
 
     def plaquette(self) -> np.ndarray:
         out = np.zeros(4 * self.n)

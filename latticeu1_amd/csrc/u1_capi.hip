@@ -386,12 +386,19 @@ struct u1_ensemble {
   double* partials = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // sweep with the plaquette summed in its last pass (u1_ensemble_sweep_plaquette)
+  double* mrows = nullptr;      // 3 segments x upd_blocks rows x 4 doubles
+  double* series_dev = nullptr;  // kEnsSeriesCap sweeps x nrep x 4 doubles
+  uint32_t* base_dev = nullptr;  // sweep index of series row 0
+  hipGraph_t mgraph = nullptr;
+  hipGraphExec_t mexec = nullptr;
   Cfg cfg = {256, 1, 1};  // launch shape of the update phases (tuned on first use)
   bool tuned = false;
 };
 
 namespace {
 
+const int kEnsSeriesCap = 256;  // sweeps per device-side plaquette series chunk
 const int kEnsBlock = 64;  // measurement launches: workgroup size and sites per thread
 const int kEnsSpt = 1;     // (small plane chunks suit the small replicas of a batch)
 
@@ -418,7 +425,13 @@ int ensemble_build_update(u1_ensemble* e, Cfg c) {
   return U1_OK;
 }
 
-int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
+// mrows (nullable): plaquette rows of the measuring phases (parity 1, mu >= 1
+// of the last pass), segment mu - 1 at row offset (mu - 1) * upd_blocks.
+int ensemble_enqueue_phases(u1_ensemble* e, void* lat, double* mrows = nullptr) {
+  auto meas_rows = [&](bool last_pass, int parity, int mu) -> double* {
+    if (!mrows || !last_pass || parity != 1 || mu < 1) return nullptr;
+    return mrows + 4 * (size_t)e->upd_blocks * (size_t)(mu - 1);
+  };
   PhaseArgs a{};
   a.reps = e->upd_dev;
   a.blockmap = e->upd_map;
@@ -437,6 +450,7 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)p;
         a.parity = parity;
+        a.meas = meas_rows(e->ovrn == 0 && p == e->metrop - 1, parity, mu);
         HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_METROPOLIS, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
                                  G.st));
       }
@@ -445,6 +459,7 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat) {
       for (int mu = 0; mu < 4; ++mu) {
         a.pass = (uint32_t)o;
         a.parity = parity;
+        a.meas = meas_rows(o == e->ovrn - 1, parity, mu);
         HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_OVERRELAX, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
                                  G.st));
       }
@@ -495,6 +510,11 @@ int ensemble_tune(u1_ensemble* e) {
 void ensemble_release(u1_ensemble* e) {
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->mexec) (void)hipGraphExecDestroy(e->mexec);
+  if (e->mgraph) (void)hipGraphDestroy(e->mgraph);
+  if (e->mrows) (void)hipFree(e->mrows);
+  if (e->series_dev) (void)hipFree(e->series_dev);
+  if (e->base_dev) (void)hipFree(e->base_dev);
   if (e->upd_dev) (void)hipFree(e->upd_dev);
   if (e->meas_dev) (void)hipFree(e->meas_dev);
   if (e->upd_map) (void)hipFree(e->upd_map);
@@ -1069,6 +1089,50 @@ int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
       RET(ensemble_enqueue_sweep(e));
   }
   e->sweeps_done += (uint64_t)nsweeps;
+  return U1_OK;
+}
+
+int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series) {
+  REQ_INIT();
+  if (!e || !series) return fail(U1_ERR_ARG, "null argument");
+  if (nsweeps < 0) return fail(U1_ERR_ARG, "nsweeps must be >= 0");
+  if (e->metrop + e->ovrn == 0) return fail(U1_ERR_ARG, "no measuring pass");
+  if (nsweeps == 0) return U1_OK;
+  if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
+  if (!e->mexec) {
+    HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
+    // one sweep: phases (measuring in the last pass), per-replica fold into
+    // the series row of this sweep, sweep index + 1
+    HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
+    int rc = ensemble_enqueue_phases(e, e->lat, e->mrows);
+    if (rc == U1_OK) {
+      hipError_t f = u1d::launch_ensemble_plaquette_fold(e->mrows, 3, e->upd_blocks, e->upd_dev, e->nrep, e->sweep_dev,
+                                                         e->base_dev, kEnsSeriesCap, e->series_dev, G.st);
+      if (f == hipSuccess) f = u1d::launch_increment(e->sweep_dev, G.st);
+      if (f != hipSuccess) rc = fail(U1_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(f));
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(G.st, &gr);
+    if (rc != U1_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPCHK(ce);
+    e->mgraph = gr;
+    HIPCHK(hipGraphInstantiate(&e->mexec, e->mgraph, nullptr, nullptr, 0));
+  }
+  for (int32_t done = 0; done < nsweeps;) {
+    const int32_t n = std::min<int32_t>(kEnsSeriesCap, nsweeps - done);
+    HIPCHK(hipMemcpyAsync(e->base_dev, e->sweep_dev, sizeof(uint32_t), hipMemcpyDeviceToDevice, G.st));
+    for (int32_t s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(e->mexec, G.st));
+    HIPCHK(hipMemcpyAsync(series + 4 * (size_t)e->nrep * done, e->series_dev, 4 * (size_t)e->nrep * n * sizeof(double),
+                          hipMemcpyDeviceToHost, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+    done += n;
+    e->sweeps_done += (uint64_t)n;
+  }
   return U1_OK;
 }
 

@@ -24,7 +24,7 @@ struct PhaseArgs {
   const uint32_t* sweep_dev;    // if set, sweep index = *sweep_dev * sweep_mul + sweep (graph replays)
   uint32_t sweep_mul;
   unsigned long long* accepts;  // if set, accepted hits are added here
-  double* meas;                 // if set (mu >= 1, single lattice): plaquette rows, 4 doubles per workgroup
+  double* meas;                 // if set (mu >= 1): plaquette rows, 4 doubles per (logical) workgroup
 };
 
 enum { KIND_METROPOLIS = 0, KIND_OVERRELAX = 1 };
@@ -86,6 +86,12 @@ hipError_t launch_plaquette_batched(int prec, const void* lat_base, const RepDes
 hipError_t launch_batched_finish(const double* partials, const RepDesc* reps, int nrep, double* out /*4*nrep*/,
                                  hipStream_t st);
 hipError_t launch_increment(uint32_t* counter, hipStream_t st);
+// Per-replica fold of the measuring phases' plaquette rows (nseg segments of
+// seg_rows rows; replica r owns rows [blk_begin, blk_begin + nblk) of each)
+// into series[(*sweep_dev - *base_dev) * nrep + r][4], skipped at or past cap.
+hipError_t launch_ensemble_plaquette_fold(const double* rows, int nseg, int seg_rows, const RepDesc* reps, int nrep,
+                                          const uint32_t* sweep_dev, const uint32_t* base_dev, int cap,
+                                          double* series, hipStream_t st);
 
 // self test (u1_selftest.hip)
 hipError_t launch_selftest_math(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t st);

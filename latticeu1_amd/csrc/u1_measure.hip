@@ -484,6 +484,32 @@ __global__ __launch_bounds__(256) void k_batched_finish(const double* __restrict
   }
 }
 
+// Per-replica fold of the plaquette rows written by the measuring update
+// phases of a batched sweep (rows of all segments in order), stored at the
+// series row of the current sweep.
+__global__ __launch_bounds__(256) void k_ensemble_plaquette_fold(const double* __restrict__ rows, int nseg,
+                                                                 int seg_rows, const RepDesc* __restrict__ reps,
+                                                                 int nrep, const uint32_t* sweep_dev,
+                                                                 const uint32_t* base_dev, int cap,
+                                                                 double* __restrict__ series) {
+  const RepDesc& r = reps[blockIdx.x];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int sgi = 0; sgi < nseg; ++sgi)
+    for (int i = threadIdx.x; i < r.nblk; i += blockDim.x) {
+      const double* row = rows + 4 * ((int64_t)sgi * seg_rows + r.blk_begin + i);
+      for (int k = 0; k < 4; ++k) v[k] += row[k];
+    }
+  double o[4];
+  block_sum<4>(v, o);
+  if (threadIdx.x == 0) {
+    const uint32_t idx = *sweep_dev - *base_dev;
+    if (idx < (uint32_t)cap) {
+      const double n = 1.0 / (3.0 * (double)r.V);
+      for (int k = 0; k < 4; ++k) series[4 * ((int64_t)idx * nrep + blockIdx.x) + k] = o[k] * n;
+    }
+  }
+}
+
 inline int nblocks(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace
@@ -518,6 +544,13 @@ hipError_t launch_plaquette_batched(int prec, const void* lat_base, const RepDes
     return plaquette_spt<double, true>(spt, (const double*)lat_base, dummy, reps, blockmap, partials, nblk, block,
                                        st);
   return plaquette_spt<float, true>(spt, (const float*)lat_base, dummy, reps, blockmap, partials, nblk, block, st);
+}
+
+hipError_t launch_ensemble_plaquette_fold(const double* rows, int nseg, int seg_rows, const RepDesc* reps, int nrep,
+                                          const uint32_t* sweep_dev, const uint32_t* base_dev, int cap,
+                                          double* series, hipStream_t st) {
+  k_ensemble_plaquette_fold<<<nrep, 256, 0, st>>>(rows, nseg, seg_rows, reps, nrep, sweep_dev, base_dev, cap, series);
+  return hipGetLastError();
 }
 
 hipError_t launch_batched_finish(const double* partials, const RepDesc* reps, int nrep, double* out, hipStream_t st) {

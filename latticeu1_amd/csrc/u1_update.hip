@@ -130,8 +130,8 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // transcendental work, so each wave keeps SPT x 19 loads in flight.  Threads
 // past the end of a plane are pointed at site 0 (in bounds) and do not store.
 //
-// MEAS (single lattice, MU >= 1, parity 1 of the last pass of a sweep): the
-// workgroup also writes row L of a.meas with its sums over the plaquettes of
+// MEAS (MU >= 1, parity 1 of the last pass of a sweep): the logical
+// workgroup L also writes row L of a.meas with its sums over the plaquettes of
 // the planes (nu, MU), nu < MU, after the update -- [sum (1 - cos P), sum
 // -sin P] in the space-space (MU < 3) or space-time (MU = 3) slot.  The three
 // phases (1, 1), (1, 2), (1, 3) come after every link of those planes has
@@ -250,12 +250,12 @@ __global__ __launch_bounds__(256) void k_hotstart(Real* __restrict__ lat, RepDes
 template <typename Real, int KIND, bool BATCHED, int SPT>
 hipError_t launch_mu(int mu, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
   if (a.meas != nullptr) {
-    // measuring phases: single lattice, mu >= 1, no multilevel mask
-    if (BATCHED || mu < 1 || a.ml != 0) return hipErrorInvalidValue;
+    // measuring phases: mu >= 1, no multilevel mask
+    if (mu < 1 || a.ml != 0) return hipErrorInvalidValue;
     switch (mu) {
-      case 1: k_phase<Real, 1, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
-      case 2: k_phase<Real, 2, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
-      default: k_phase<Real, 3, KIND, false, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+      case 1: k_phase<Real, 1, KIND, BATCHED, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+      case 2: k_phase<Real, 2, KIND, BATCHED, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
+      default: k_phase<Real, 3, KIND, BATCHED, SPT, true><<<nblocks, block, 0, s>>>(lat, a); break;
     }
     return hipGetLastError();
   }

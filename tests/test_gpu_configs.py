@@ -30,14 +30,6 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
 
-def jackknife(x, nbins=20):
-    x = np.asarray(x)
-    n = len(x) // nbins * nbins
-    b = x[:n].reshape(nbins, -1, *x.shape[1:]).mean(axis=1)
-    jk = (b.sum(axis=0) - b) / (nbins - 1)
-    return x[:n].mean(axis=0), np.sqrt((nbins - 1) / nbins * ((jk - jk.mean(axis=0)) ** 2).sum(axis=0))
-
-
 # Multihit workload: "xi = 2 (beta_s = 1, beta_t = beta_s * xi)": the reference weights
 # space-space staples by beta/xi and space-time by beta*xi (staple.h:75-76),
 # so beta_s = 1, beta_t = 2 means beta = sqrt(2), xi = sqrt(2).
@@ -182,6 +174,9 @@ def test_ensemble_64_uneven_fp64_parity(u1, orc):
 
 
 def test_ensemble_64_uneven_fp32_vs_fp64(u1):
+    """Timed: 300 graph sweeps of all 64 replicas with every replica's
+    plaquette summed inside each sweep (u1_ensemble_sweep_plaquette); the
+    series gives the per-replica <P> compared between precisions."""
     dims, betas, hot, seeds = ensemble_specs()
     geoms = [u1.geom(d, b) for d, b in zip(dims, betas)]
     n_links = sum(4 * int(np.prod(d)) for d in dims)
@@ -189,22 +184,28 @@ def test_ensemble_64_uneven_fp32_vs_fp64(u1):
     for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
         e = u1.Ensemble(geoms, seeds, hot, prec, 1, 3, u1.BENCH_METRO)
         e.sweep(200, graph=True)  # thermalisation
+        e.sweep_plaquette(1)      # builds the measuring graph outside the timed region
         u1.sync()
         t0 = time.perf_counter()
-        e.sweep(300, graph=True)
-        u1.sync()
+        ser = e.sweep_plaquette(300)
         el = time.perf_counter() - t0
-        say(f"ensemble {name}: 64 replicas, {n_links} links, 300 graph sweeps in {el:.3f} s -> "
-            f"{4 * n_links * 300 / el:.3e} link updates/s")
-        vals = []
-        for s in range(200):
-            e.sweep(1, graph=True)
-            vals.append([u1.mean_plaquette(row) for row in e.plaquette()])
-        series[name] = np.array(vals)
+        say(f"ensemble {name}: 64 replicas, {n_links} links, 300 graph sweeps with per-sweep plaquettes in "
+            f"{el:.3f} s -> {4 * n_links * 300 / el:.3e} link updates/s")
+        series[name] = np.array([[u1.mean_plaquette(row) for row in sw] for sw in ser])
         e.destroy()
-    m32, e32 = jackknife(series["fp32"], 20)
-    m64, e64 = jackknife(series["fp64"], 20)
-    z = (m32 - m64) / np.hypot(e32, e64)
-    say(f"ensemble per-replica <P> FP32 - FP64 in combined sigma: mean |z| {np.mean(np.abs(z)):.2f}, max |z| "
-        f"{np.max(np.abs(z)):.2f}, fraction within 1 sigma {np.mean(np.abs(z) < 1):.2f}")
-    assert np.max(np.abs(z)) < 5.0
+    from latticeu1_amd.analysis import mean_error
+
+    z = []
+    for r in range(len(dims)):
+        m32, e32 = mean_error(series["fp32"][:, r])
+        m64, e64 = mean_error(series["fp64"][:, r])
+        z.append((m32 - m64) / math.hypot(e32, e64))
+    z = np.array(z)
+    say(f"ensemble per-replica <P> FP32 - FP64 in combined sigma (Gamma-method errors): median |z| "
+        f"{np.median(np.abs(z)):.2f}, max |z| {np.max(np.abs(z)):.2f}, within 1 sigma {np.mean(np.abs(z) < 1):.2f}, "
+        f"within 2 sigma {np.mean(np.abs(z) < 2):.2f}")
+    # 300 sweeps per chain; replicas with beta near the transition (~1.01)
+    # tunnel slowly, so the bar is robust: a broken FP32 path shifts the bulk
+    assert np.median(np.abs(z)) < 1.2
+    assert np.mean(np.abs(z) < 2.0) >= 0.8
+    assert np.max(np.abs(z)) < 8.0

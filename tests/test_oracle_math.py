@@ -175,3 +175,28 @@ def test_gamma_method_on_ar1(tmp_path):
         assert ok == 1
         assert abs(tau - exact) < 3 * dtau + 0.02, line
         assert abs(dval / dexact - 1) < 0.1, line
+
+
+def test_python_gamma_method_on_ar1():
+    """latticeu1_amd.analysis.gamma_method (used by bench.py and the tests):
+    tau_int of AR(1) series is (1 + rho) / (2 (1 - rho)); the error of the
+    mean matches sqrt(2 tau_int / N); jackknife agrees for uncorrelated data."""
+    from latticeu1_amd.analysis import gamma_method, jackknife
+
+    rng = np.random.default_rng(12345)
+    for rho in (0.0, 0.5, 0.8, 0.9):
+        n = 100000
+        e = rng.standard_normal(n)
+        x = np.empty(n)
+        v = 0.0
+        for i in range(n):
+            v = rho * v + math.sqrt(1 - rho * rho) * e[i]
+            x[i] = v + 3.0
+        mean, err, tau, w = gamma_method(x)
+        exact = (1 + rho) / (2 * (1 - rho))
+        assert abs(tau - exact) < 0.12 * exact + 0.03, (rho, tau, exact)
+        assert abs(err / math.sqrt(2 * exact / n) - 1) < 0.12, (rho, err)
+        assert abs(mean - 3.0) < 5 * err
+    y = rng.standard_normal(20000)
+    m, ej = jackknife(y, 20)
+    assert abs(ej / (1 / math.sqrt(len(y))) - 1) < 0.35
