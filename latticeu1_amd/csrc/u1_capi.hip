@@ -1071,7 +1071,8 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
 int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
   REQ_INIT();
   if (!e) return fail(U1_ERR_ARG, "null ensemble");
-  if (G.tune && !e->tuned && !e->exec) RET(ensemble_tune(e));
+  // tuning changes the launch layout, so it only runs before either graph exists
+  if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
   if (use_graph && !e->exec) {
     HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
     int rc = ensemble_enqueue_sweep(e);
@@ -1100,9 +1101,11 @@ int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series)
   if (nsweeps == 0) return U1_OK;
   if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
   if (!e->mexec) {
-    HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
+    // (kept across a failed capture and reused by the next attempt)
+    if (!e->mrows) HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
+    if (!e->series_dev)
+      HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
+    if (!e->base_dev) HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
     // one sweep: phases (measuring in the last pass), per-replica fold into
     // the series row of this sweep, sweep index + 1
     HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
