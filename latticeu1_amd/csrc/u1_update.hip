@@ -44,6 +44,13 @@ struct PlaqPieces {
   Real ur, ui, lr, li;
 };
 
+// First direction nu != mu of the space-time (st) or space-space class.
+__host__ __device__ constexpr int first_nu(int mu, bool st) {
+  for (int nu = 0; nu < 4; ++nu)
+    if (nu != mu && ((mu == 3 || nu == 3) == st)) return nu;
+  return -1;
+}
+
 // Staple of link (x, MU) (staple.h:79-102), with the operation order of the
 // oracle.  fwd/bwd[d] are the eo-id offsets to x +- e_d.  With MEAS the
 // plaquette pieces are collected on the side (the staple is unchanged).
@@ -75,13 +82,21 @@ __device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, 
       pp->lr += cl;
       pp->li += sl;
     }
-    if (MU == 3 || nu == 3) {
-      str += ar;
-      sti += ai;
+    // binary32: the first term of each class initialises its sum (the oracle's
+    // 0 + x is kept in binary64, where the kernels match it bit for bit)
+    const bool st_class = MU == 3 || nu == 3;
+    const bool first = sizeof(Real) == sizeof(float) && nu == first_nu(MU, st_class);
+    if (st_class) {
+      str = first ? ar : str + ar;
+      sti = first ? ai : sti + ai;
     } else {
-      ssr += ar;
-      ssi += ai;
+      ssr = first ? ar : ssr + ar;
+      ssi = first ? ai : ssi + ai;
     }
+  }
+  if (sizeof(Real) == sizeof(float) && MU == 3) {  // no space-space staples
+    if (!aniso) return {str, sti};
+    return {str * xi, sti * xi};
   }
   if (!aniso) return {ssr + str, ssi + sti};  // xi == 1: identical to ss/1 + st*1
   if (sizeof(Real) == sizeof(double)) return {ssr / xi + str * xi, ssi / xi + sti * xi};
@@ -202,7 +217,8 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     for (int k = 0; k < SPT; ++k) {
       // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped in FP64
       const Real t2 = UMath<Real>::atan2(S[k].im, S[k].re);
-      th[k] = (Real)u1m::wrap_d(-(double)th[k] - 2.0 * (double)t2);
+      const double x = -(double)th[k] - 2.0 * (double)t2;
+      th[k] = (Real)(sizeof(Real) == sizeof(double) ? u1m::wrap_d(x) : u1m::wrap3_d(x));
     }
   }
 #pragma unroll

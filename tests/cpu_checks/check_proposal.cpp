@@ -10,6 +10,8 @@
 #include <random>
 #include <vector>
 
+#include "u1_math.h"
+
 static double two_step(uint32_t w, double a) {
   const double u = (double)w * 0x1p-32;
   return a * (2.0 * u - 1.0);
@@ -38,5 +40,36 @@ int main() {
   std::printf("%ld of %ld differ\n", bad, n);
   if (bad) return 1;
   std::printf("proposal ok\n");
+
+  // wrap3_d (binary32 overrelaxation) against the stepwise wrap_d on
+  // [-3 pi - 1e-6, 3 pi + 1e-6]: range [-pi, pi), within 1 ulp of binary64
+  long wbad = 0, wn = 0;
+  auto wcheck = [&](double x) {
+    const double a = u1m::wrap_d(x), b = u1m::wrap3_d(x);
+    ++wn;
+    const bool in = b >= -u1m::K_PI && b < u1m::K_PI;
+    const double ulp = std::nextafter(std::fabs(a), INFINITY) - std::fabs(a);
+    if (!in || std::fabs(a - b) > ulp) {
+      if (wbad < 5) std::printf("wrap mismatch x=%.17g: %.17g vs %.17g\n", x, a, b);
+      ++wbad;
+    }
+  };
+  const double lim = 3.0 * u1m::K_PI + 1e-6;
+  std::uniform_real_distribution<double> ux(-lim, lim);
+  for (int i = 0; i < (1 << 22); ++i) wcheck(ux(rng));
+  for (int m = -3; m <= 3; ++m)  // odd multiples of pi and their neighbours
+    for (double base : {m * u1m::K_PI, m * u1m::K_PI + 1e-6, m * u1m::K_PI - 1e-6}) {
+      double x = base;
+      for (int s = 0; s < 64; ++s) x = std::nextafter(x, INFINITY);
+      double y = base;
+      for (int s = 0; s < 128; ++s) {
+        wcheck(y);
+        y = std::nextafter(y, INFINITY);
+      }
+      (void)x;
+    }
+  std::printf("wrap: %ld of %ld out of range or beyond 1 ulp\n", wbad, wn);
+  if (wbad) return 1;
+  std::printf("wrap ok\n");
   return 0;
 }

@@ -28,14 +28,16 @@ def test_device_indexing_on_cpu(tmp_path):
 
 def test_fma_proposal_matches_two_step_form(tmp_path):
     """The kernels' one-FMA Metropolis proposal offset is bit-identical to the
-    oracle's two-step a (2u - 1) (tests/cpu_checks/check_proposal.cpp)."""
+    oracle's two-step a (2u - 1), and the binary32 overrelaxation's wrap3_d
+    lands in [-pi, pi) within 1 ulp of wrap_d (tests/cpu_checks/check_proposal.cpp)."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
     exe = tmp_path / "check_proposal"
-    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-mfma",
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-mfma", "-I",
+                    os.path.join(os.path.dirname(HERE), "latticeu1_amd", "csrc"),
                     os.path.join(HERE, "cpu_checks", "check_proposal.cpp"), "-o", str(exe)],
                    check=True, capture_output=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "proposal ok" in out.stdout
+    assert "proposal ok" in out.stdout and "wrap ok" in out.stdout
