@@ -250,6 +250,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   v += dpp_f64<0x140>(v);
   return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
 }
+// Integer sum over the wave (accept counters; order-independent).
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

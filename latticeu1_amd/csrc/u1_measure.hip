@@ -2,9 +2,9 @@
 // analytic and stochastic multihit), correlator PP, layout conversions and
 // plaquette fields.
 //
-// Reductions are deterministic: every workgroup writes its partial sums
-// (wave64 butterfly, then waves in order through LDS) to a partials buffer and
-// k_finish folds the partials in workgroup order -- replacing the reference's
+// Reductions are deterministic: every workgroup writes its partial sums (DPP
+// row sums within each wave64, then waves in order through LDS) to a partials
+// buffer, and k_fold_rows / k_finish fold the rows in order -- replacing the reference's
 // shared-memory fold plus one FP64 atomicAdd per block (reduce_block_1d.h:41-64,
 // cudaAtomic.h:147-177), whose summation order is nondeterministic.  All sums
 // are FP64 for both lattice precisions.
