@@ -55,6 +55,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 #define REQ_INIT() \
   if (!G.init) return fail(U1_ERR_STATE, "u1_init has not been called")
+// Pointers handed to kernels or written on the host are never null.
+#define REQ_PTR(p) \
+  if (!(p)) return fail(U1_ERR_ARG, "null pointer argument: " #p)
 #define RET(expr)             \
   do {                        \
     int rc_ = (expr);         \
@@ -611,6 +614,7 @@ int u1_memcpy(void* dst, const void* src, size_t bytes, int kind) {
 
 int u1_zero(void* ptr, size_t bytes) {
   REQ_INIT();
+  if (bytes) REQ_PTR(ptr);
   if (bytes % 4) return fail(U1_ERR_ARG, "u1_zero needs a multiple of 4 bytes");
   HIPCHK(u1d::launch_fill_zero(ptr, (int64_t)bytes, G.st));
   return U1_OK;
@@ -618,6 +622,9 @@ int u1_zero(void* ptr, size_t bytes) {
 
 int u1_convert_precision(const void* src, int src_prec, void* dst, int dst_prec, int64_t n) {
   REQ_INIT();
+  if (n < 0) return fail(U1_ERR_ARG, "negative element count");
+  if (n) REQ_PTR(src);
+  if (n) REQ_PTR(dst);
   RET(check_prec(src_prec));
   RET(check_prec(dst_prec));
   HIPCHK(u1d::launch_convert_precision(src, src_prec, dst, dst_prec, n, 256, G.st));
@@ -627,6 +634,7 @@ int u1_convert_precision(const void* src, int src_prec, void* dst, int dst_prec,
 // ---------------------------------------------------------------- updates
 int u1_hotstart(const u1_geom* g, int prec, void* lat, uint64_t seed, uint32_t replica) {
   REQ_INIT();
+  REQ_PTR(lat);
   RET(check_geom(g));
   RET(check_prec(prec));
   const RepDesc r = make_rep(g, seed, replica);
@@ -697,6 +705,8 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
 // ---------------------------------------------------------------- measurements
 int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out);
   RET(check_geom(g));
   RET(check_prec(prec));
   RET(plaquette_async(g, prec, lat, G.dres));
@@ -705,6 +715,8 @@ int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]) {
 
 int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out);
   RET(check_geom(g));
   RET(check_prec(prec));
   const RepDesc r = make_rep(g, 0, 0);
@@ -718,6 +730,8 @@ int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]) {
 int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
                    uint32_t stream, uint32_t c1, uint32_t c3, void* P_dev) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(P_dev);
   RET(check_geom(g));
   RET(check_prec(prec));
   if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad Polyakov mode");
@@ -741,6 +755,8 @@ int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, co
 int u1_temporal_links(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh,
                       uint64_t seed, uint32_t stream, uint32_t c1, uint32_t c3, void* out_dev) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out_dev);
   RET(check_geom(g));
   RET(check_prec(prec));
   if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad temporal-link mode");
@@ -753,6 +769,8 @@ int u1_temporal_links(const u1_geom* g, int prec, const void* lat, int32_t mode,
 
 int u1_pp(const u1_geom* g, const void* P_dev, int32_t rmax, double* out) {
   REQ_INIT();
+  REQ_PTR(P_dev);
+  REQ_PTR(out);
   RET(check_geom(g));
   if (rmax < 1 || rmax > 32) return fail(U1_ERR_ARG, "rmax must be in [1, 32]");
   const RepDesc r = make_rep(g, 0, 0);
@@ -767,6 +785,8 @@ int u1_pp(const u1_geom* g, const void* P_dev, int32_t rmax, double* out) {
 int u1_poly2(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh, uint64_t seed,
              uint32_t c1, uint32_t c3, double* out) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out);
   RET(check_geom(g));
   const int64_t sv = (int64_t)g->dims[0] * g->dims[1] * g->dims[2];
   void* P = nullptr;
@@ -782,6 +802,8 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
                   int32_t metrop, int32_t ovrn, const u1_metro* m, int32_t mh_mode, const u1_metro* mh,
                   uint64_t seed, uint32_t call_id, double* out, void* ppspace_dev) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out);
   RET(check_geom(g));
   RET(check_prec(prec));
   RET(check_metro(m));
@@ -896,6 +918,8 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
 
 int u1_convert_layout(const u1_geom* g, int prec, const void* in, void* out, int32_t eo_to_no) {
   REQ_INIT();
+  REQ_PTR(in);
+  REQ_PTR(out);
   RET(check_geom(g));
   RET(check_prec(prec));
   if (in == out) return fail(U1_ERR_ARG, "in and out must differ");
@@ -907,6 +931,9 @@ int u1_convert_layout(const u1_geom* g, int prec, const void* in, void* out, int
 int u1_plaquette_fields(const u1_geom* g, int prec, const void* lat, int32_t spacetime, int32_t eo_order,
                         void* field_dev, double means[12]) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(field_dev);
+  REQ_PTR(means);
   RET(check_geom(g));
   RET(check_prec(prec));
   const RepDesc r = make_rep(g, 0, 0);
@@ -1273,6 +1300,8 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms) {
   REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(elapsed_ms);
   RET(check_geom(g));
   RET(check_prec(prec));
   RET(check_metro(m));

@@ -303,6 +303,32 @@ def test_multilevel_errors(u1):
         bad.plaquette()  # ...but odd extents are rejected (parameters.cu:163-168)
 
 
+def test_geometry_and_argument_limits(u1):
+    """Geometry and argument checks run before any device work: odd or tiny
+    extents (parameters.cu:163-168), 4V >= 2^31 (32-bit byte offsets within a
+    direction plane), bad proposals, nhit and pass counts all fail cleanly."""
+    import ctypes as C
+
+    out = np.zeros(4)
+    for dims in [(4, 4, 3, 4), (0, 4, 4, 4), (152, 152, 152, 152), (2, 2, 2, 1)]:
+        with pytest.raises(u1.U1Error) as e:
+            u1.call("u1_plaquette", C.byref(u1.geom(dims)), u1.F32, None, u1._dp(out))
+        assert e.value.code == 1, dims
+    g = u1.geom((150, 150, 150, 150))  # 4V = 2.0e9 < 2^31: geometry accepted, null lattice rejected
+    with pytest.raises(u1.U1Error):
+        u1.call("u1_plaquette", C.byref(g), u1.F32, None, u1._dp(out))
+    ok = u1.geom((4, 4, 4, 4))
+    lat = u1.Lattice.cold(ok, u1.F32)
+    for m in [u1.metro(7, 4, 0.5), u1.metro(1, 0, 0.5), u1.metro(1, 4, 4.0), u1.metro(1, 9000, 0.5)]:
+        with pytest.raises(u1.U1Error):
+            lat.update(1, 3, m, 1, 0)
+    with pytest.raises(u1.U1Error):
+        lat.update(-1, 3, u1.BENCH_METRO, 1, 0)
+    with pytest.raises(u1.U1Error):
+        lat.update_plaquette(0, 0, u1.BENCH_METRO, 1, 0)
+    assert np.all(lat.to_host() == 0.0)  # nothing ran
+
+
 def test_layout_and_fields_fp64(u1, orc, thermal):
     g, og, lat, ref = thermal
     no = lat.convert_layout(True)
