@@ -54,3 +54,48 @@ def test_dropin_driver_matches_oracle(orc, tmp_path):
         rows = (tmp_path / fn).read_text().strip().splitlines()
         assert len(rows) == 4 and rows[0].split("\t")[0] == "1"
     assert (tmp_path / "tunecache.csv").read_text().startswith("tunecache")
+
+
+RUN = os.path.join(ROOT, "latticeu1_amd", "u1_run")
+
+
+def test_run_driver_files_match_oracle(orc, tmp_path):
+    """latticeu1_amd/u1_run, the reference's u1.cpp loop on the drop-in layer:
+    plaq_/poly_ files every printiter sweeps (line formats of u1.cpp:84-128),
+    and Poly2 / multihit / MultiLevel files from iteration 1000 on.  The FP64
+    chain is the oracle's (default proposal: independent, 1 hit)."""
+    assert os.path.exists(RUN), "build() has not produced latticeu1_amd/u1_run"
+    print("[gpu] run driver 8^4, 1000 sweeps", flush=True)
+    out = subprocess.run([RUN, "8", "8", "8", "8", "1.0", "1.0", "1", "3", "1000", "250", "1"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    name = "8_8_8_8_beta_1_xi_1_mtr_1_ovr_3"
+    plaq = [line.split("\t") for line in (tmp_path / f"plaq_{name}.dat").read_text().strip().splitlines()]
+    poly = [line.split("\t") for line in (tmp_path / f"poly_{name}.dat").read_text().strip().splitlines()]
+    assert [int(r[0]) for r in plaq] == [0, 250, 500, 750, 1000]
+    assert [int(r[0]) for r in poly] == [0, 250, 500, 750, 1000]
+    og = orc.geom((8, 8, 8, 8), 1.0, 1.0)
+    m = orc.metro(0, 1, 0.5)
+    ref = orc.hotstart(og, 1234)
+    k = 0
+    for it in range(1001):
+        if it > 0:
+            orc.update(og, ref, 1, 3, m, 1234, it - 1)
+        if it % 250 == 0:
+            p = orc.plaquette(og, ref)
+            assert np.allclose([float(v) for v in plaq[k][1:5]], p, rtol=1e-10, atol=1e-12), (it, plaq[k], p)
+            L = orc.polyakov(og, ref)
+            assert len(poly[k]) == 4 and poly[k][3].startswith("|L|: ")
+            assert abs(complex(float(poly[k][1]), float(poly[k][2])) - L) < 1e-11, (it, poly[k], L)
+            assert abs(float(poly[k][3][5:]) - abs(L)) < 1e-11
+            k += 1
+    c0 = orc.poly2(og, ref, 0)
+    rows = (tmp_path / f"Pot_{name}_iter_1000.dat").read_text().strip().splitlines()
+    assert len(rows) == 4
+    for r, row in enumerate(rows):
+        f = row.split("\t")
+        assert int(f[0]) == r + 1
+        assert abs(complex(float(f[1]), float(f[2])) - c0[r]) < 1e-11 * max(1.0, abs(c0[r]))
+    for fn in (f"Pot_mhit_{name}_iter_1000.dat", f"Pot_mlevel_{name}_iter_1000_10_1_20_1_2_5.dat"):
+        rows = (tmp_path / fn).read_text().strip().splitlines()
+        assert len(rows) == 4 and rows[0].split("\t")[0] == "1", fn
