@@ -135,6 +135,29 @@ def test_fp64_parity_geometries(u1, orc, dims, beta, aniso, metrop, ovrn, nsw):
     assert np.max(np.abs(lat.plaquette() - orc.plaquette(og, ref))) <= 1e-12
 
 
+@pytest.mark.parametrize("dims", [(24, 24, 24, 24), (48, 8, 6, 10)])
+def test_fp64_parity_multichunk_planes(u1, orc, dims):
+    """Planes of P = 288 and 192 even/odd sites, split into several workgroup
+    chunks (a ragged tail for most tuned shapes; workgroup counts that need not
+    divide by the 8 XCDs): 8 sweeps bit-identical to the oracle with identical
+    accept counts, then a fused-plaquette sweep."""
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=31)
+    say(f"fp64 parity multi-chunk {dims}")
+    m = u1.BENCH_METRO
+    om = orc.metro(1, 4, 0.5)
+    for s in range(8):
+        assert lat.update(1, 3, m, 31, s, count_accepts=True) == \
+            orc.update(og, ref, 1, 3, om, 31, s, count_accepts=True), s
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    assert np.max(np.abs(lat.update_plaquette(1, 3, m, 31, 8) - orc.plaquette(og, orc_step(orc, og, ref, om, 31, 8)))) <= 1e-12
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+
+
+def orc_step(orc, og, ref, om, seed, sweep):
+    orc.update(og, ref, 1, 3, om, seed, sweep)
+    return ref
+
+
 @pytest.mark.parametrize("dims", [
     (2, 2, 2, 2),      # P = 2: one mostly idle workgroup per plane
     (10, 6, 4, 6),     # P = 30, ragged extents
