@@ -284,6 +284,8 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
                uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
                const GroupEvents* gev = nullptr, double* meas_dst = nullptr, const uint32_t* sweep_dev = nullptr,
                uint32_t sweep_mul = 0) {
+  // the Metropolis pass index takes 12 bits of Philox counter word 2
+  if (metrop < 0 || ovrn < 0 || metrop > 4095) return fail(U1_ERR_ARG, "bad pass counts");
   PhaseArgs a{};
   a.single = make_rep(g, seed, c3);
   a.reps = nullptr;
@@ -1030,6 +1032,7 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
                        const int32_t* hot, int prec, int32_t metrop, int32_t ovrn, const u1_metro* m) {
   REQ_INIT();
   if (!out || !geoms || !seeds || nrep < 1 || nrep > 65535) return fail(U1_ERR_ARG, "bad ensemble arguments");
+  if (metrop < 0 || ovrn < 0 || metrop > 4095 || ovrn > 4095) return fail(U1_ERR_ARG, "bad pass counts");
   RET(check_prec(prec));
   RET(check_metro(m));
   *out = nullptr;
