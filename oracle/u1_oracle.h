@@ -10,6 +10,12 @@
  * same Philox4x32-10 streams and the same elementary functions (u1_math.h) as
  * the GPU path, so that the FP64 GPU path must reproduce it sweep by sweep.
  *
+ * Pinning: every function here is checked in tests/ against a closed form, a
+ * known-answer vector (Philox), an independent numpy restatement or a library
+ * routine (glibc, scipy).  Parity unpinned: the trajectory of the reference
+ * itself, which draws from cuRAND MRG32k3a and CUDA's cyl_bessel_i0/i1 (closed
+ * code, no golden vectors in the reference); agreement with it is statistical.
+ *
  * Lattice layout (reference array.h / index.h:124-132): angles theta_mu(x) in
  * even/odd structure-of-arrays order  lat[eo_id + parity*V/2 + mu*V],
  * eo_id = lex(x) >> 1,  lex = x0 + Nx*(x1 + Ny*(x2 + Nz*x3)).  4D only.
