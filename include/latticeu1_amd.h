@@ -73,7 +73,7 @@ typedef struct {
 } u1_geom;
 
 /* Metropolis settings.  Reference behaviour: proposal INDEPENDENT, nhit 1
- * (update.cu:181-182).  BASELINE-style runs: DELTA, nhit 4, step 0.5. */
+ * (update.cu:181-182).  The 48^4 benchmark runs: DELTA, nhit 4, step 0.5. */
 typedef struct {
   int32_t proposal;
   int32_t nhit;
