@@ -42,3 +42,37 @@ def test_fp32_fp64_ensemble_means_agree(u1, beta):
     z = (a - b) / math.hypot(ea, eb)
     say(f"beta={beta}: <P> fp32 {a:.7f} +- {ea:.1e}, fp64 {b:.7f} +- {eb:.1e}, diff {a - b:+.2e} = {z:+.2f} sigma")
     assert abs(z) < 3.5
+
+
+def test_fp32_fp64_independent_chains_48x4(u1):
+    """The 48^4 benchmark configuration itself (beta = 1, hot start, 1 Metropolis
+    with 4 hits + 3 OR, 500 thermalisation + 1000 measured sweeps with the
+    plaquette every sweep), four independent chains per precision (different
+    seeds): the mean over chains, with Gamma-method errors per chain, agrees
+    between precisions.  beta = 1 is close to the transition, so the
+    chain-to-chain spread is reported beside the per-chain errors."""
+    from latticeu1_amd.analysis import mean_error
+
+    res = {}
+    for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
+        means, errs = [], []
+        for c in range(4):
+            seed = 20241 + 1000 * c + (0 if prec == u1.F32 else 500000)
+            lat = u1.Lattice.cold(u1.geom((48, 48, 48, 48), 1.0), prec)
+            lat.hotstart(seed)
+            lat.bench(1, 3, u1.BENCH_METRO, seed, 0, 500, 0)
+            _, _, series = lat.bench(1, 3, u1.BENCH_METRO, seed, 500, 1000, 1)
+            m, e = mean_error([u1.mean_plaquette(r) for r in series])
+            means.append(m)
+            errs.append(e)
+            del lat
+        means, errs = np.array(means), np.array(errs)
+        mean = means.mean()
+        err = max(math.sqrt(np.sum(errs ** 2)) / len(means), means.std(ddof=1) / math.sqrt(len(means)))
+        res[name] = (mean, err)
+        say(f"48^4 {name}: chain means {np.array2string(means, precision=6)} (errors "
+            f"{np.array2string(errs, precision=1)}), mean {mean:.7f} +- {err:.1e}")
+    d = res["fp32"][0] - res["fp64"][0]
+    z = d / math.hypot(res["fp32"][1], res["fp64"][1])
+    say(f"48^4 <P> FP32 - FP64 over independent chains: {d:+.2e} = {z:+.2f} combined sigma")
+    assert abs(z) < 3.0
