@@ -34,6 +34,8 @@ struct Ctx {
   hipStream_t st = nullptr;
   double* partials = nullptr;  // reduction scratch
   size_t partials_cap = 0;     // in doubles
+  double* work = nullptr;      // Polyakov-volume and temporal-link scratch of u1_poly_volume / u1_poly2
+  size_t work_cap = 0;         // in doubles
   unsigned long long* counter = nullptr;
   double* dres = nullptr;  // small device result buffer (64 doubles)
   bool tune = true;
@@ -63,6 +65,42 @@ int fail(int code, const std::string& msg) {
     int rc_ = (expr);         \
     if (rc_ != U1_OK) return rc_; \
   } while (0)
+
+// Temporary device buffer, freed on every return path (after the library
+// stream has drained, so no queued kernel still uses it).
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) {
+      (void)hipStreamSynchronize(G.st);
+      (void)hipFree(p);
+    }
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 4); }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// HIP events owned by one call, destroyed on every return path.
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  EventSet() = default;
+  EventSet(const EventSet&) = delete;
+  EventSet& operator=(const EventSet&) = delete;
+  ~EventSet() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  hipError_t create(hipEvent_t* out) {
+    const hipError_t e = hipEventCreate(out);
+    if (e == hipSuccess) ev.push_back(*out);
+    return e;
+  }
+};
 
 // parameters.cu:163-168 rejects odd extents; the kernels here also need
 // 4 * V < 2^31 (32-bit link index and Philox counter word).
@@ -136,6 +174,18 @@ int ensure_partials(size_t n) {
   return U1_OK;
 }
 
+// Kept between calls and grown on demand (the reference allocates these
+// per call, polyakov.cu:443-508, 618-654).
+int ensure_work(size_t n) {
+  if (n <= G.work_cap) return U1_OK;
+  if (G.work) HIPCHK(hipFree(G.work));
+  G.work = nullptr;
+  G.work_cap = 0;
+  HIPCHK(hipMalloc((void**)&G.work, n * sizeof(double)));
+  G.work_cap = n;
+  return U1_OK;
+}
+
 // Room for nblk rows of nv partial sums followed by the fold's scratch rows.
 int ensure_fold(int nblk, int nv) { return ensure_partials((size_t)nv * nblk + u1d::fold_scratch(nblk, nv)); }
 double* fold_scratch_ptr(int nblk, int nv) { return G.partials + (size_t)nv * nblk; }
@@ -195,9 +245,10 @@ std::string tune_key(const char* name, const u1_geom* g, int prec) {
 }
 
 int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, Cfg c, float* ms) {
+  EventSet es;
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
   HIPCHK(u1d::launch_phase(prec, kind, 0, lat, a, nb, c.block, c.spt, G.st));  // warm-up
   HIPCHK(hipEventRecord(e0, G.st));
   for (int rep = 0; rep < 2; ++rep)
@@ -205,8 +256,6 @@ int time_phase(int prec, int kind, void* lat, const PhaseArgs& a, int nb, Cfg c,
   HIPCHK(hipEventRecord(e1, G.st));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
-  HIPCHK(hipEventDestroy(e0));
-  HIPCHK(hipEventDestroy(e1));
   return U1_OK;
 }
 
@@ -224,9 +273,9 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
     return U1_OK;
   }
   const int64_t bytes = 4 * (int64_t)proto.single.V * (int64_t)elem_size(prec);
-  void* scratch = nullptr;
-  HIPCHK(hipMalloc(&scratch, bytes));
-  HIPCHK(hipMemcpyAsync(scratch, lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  DevBuf scratch;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(hipMemcpyAsync(scratch.p, lat, bytes, hipMemcpyDeviceToDevice, G.st));
   // Three rounds over all candidates, best time per candidate: a round-robin
   // order keeps clock ramp-up and neighbour noise from favouring one shape.
   std::vector<Cfg> cands;
@@ -242,14 +291,9 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
       a.xcd = (uint32_t)c.xcd;
       set_blocks(a.single, c.block * c.spt);
       float ms = 0;
-      int rc = time_phase(prec, kind, scratch, a, a.single.nblk, c, &ms);
-      if (rc != U1_OK) {
-        (void)hipFree(scratch);
-        return rc;
-      }
+      RET(time_phase(prec, kind, scratch.p, a, a.single.nblk, c, &ms));
       tbest[i] = std::min(tbest[i], ms);
     }
-  HIPCHK(hipFree(scratch));
   size_t ib = 0;
   for (size_t i = 1; i < cands.size(); ++i)
     if (tbest[i] < tbest[ib]) ib = i;
@@ -363,6 +407,21 @@ int plaquette_async(const u1_geom* g, int prec, const void* lat, double* dst) {
   return U1_OK;
 }
 
+// Polyakov-loop volume P (polyakov.cu:415-508) into P (2 SV doubles, device);
+// the multihit modes first write the temporal-link field into tl (2 V
+// doubles), then take the product along t.
+int poly_volume_async(int prec, const void* lat, const RepDesc& r, int mode, const u1_metro* mh, uint32_t stream,
+                      uint32_t c1, uint32_t c3, double* P, double* tl) {
+  if (mode == U1_POLY_PLAIN) {
+    HIPCHK(u1d::launch_poly_volume(prec, lat, r, mode, 0, 0, 0.0, stream, c1, c3, P, 256, G.st));
+    return U1_OK;
+  }
+  HIPCHK(u1d::launch_temporal_links(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0,
+                                    mh ? mh->step : 0.0, stream, c1, c3, tl, 256, G.st));
+  HIPCHK(u1d::launch_tprod(tl, r, P, G.st));
+  return U1_OK;
+}
+
 int copy_back(double* host, const double* dev, size_t n) {
   HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
@@ -393,6 +452,7 @@ struct u1_ensemble {
   hipGraphExec_t exec = nullptr;
   // sweep with the plaquette summed in its last pass (u1_ensemble_sweep_plaquette)
   double* mrows = nullptr;      // 3 segments x upd_blocks rows x 4 doubles
+  int mrows_blocks = 0;         // upd_blocks that mrows was sized for
   double* series_dev = nullptr;  // kEnsSeriesCap sweeps x nrep x 4 doubles
   uint32_t* base_dev = nullptr;  // sweep index of series row 0
   hipGraph_t mgraph = nullptr;
@@ -481,20 +541,21 @@ int ensemble_enqueue_sweep(u1_ensemble* e) {
 // scratch copy of the packed lattice; keep the fastest.
 int ensemble_tune(u1_ensemble* e) {
   const size_t bytes = (size_t)e->total_links * elem_size(e->prec);
-  void* scratch = nullptr;
-  HIPCHK(hipMalloc(&scratch, bytes));
-  HIPCHK(hipMemcpyAsync(scratch, e->lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  DevBuf scratch;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(hipMemcpyAsync(scratch.p, e->lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  EventSet es;
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
   float best = 1e30f;
   Cfg bestc = e->cfg;
   for (const Cfg c : candidates()) {
     if (c.spt == 4 && e->prec != U1_F32) continue;
     RET(ensemble_build_update(e, c));
-    RET(ensemble_enqueue_phases(e, scratch));  // warm-up
+    RET(ensemble_enqueue_phases(e, scratch.p));  // warm-up
     HIPCHK(hipEventRecord(e0, G.st));
-    RET(ensemble_enqueue_phases(e, scratch));
+    RET(ensemble_enqueue_phases(e, scratch.p));
     HIPCHK(hipEventRecord(e1, G.st));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -504,9 +565,6 @@ int ensemble_tune(u1_ensemble* e) {
       bestc = c;
     }
   }
-  HIPCHK(hipEventDestroy(e0));
-  HIPCHK(hipEventDestroy(e1));
-  HIPCHK(hipFree(scratch));
   RET(ensemble_build_update(e, bestc));
   e->tuned = true;
   return U1_OK;
@@ -556,6 +614,9 @@ int u1_finalize(void) {
   if (!G.init) return U1_OK;
   HIPCHK(hipStreamSynchronize(G.st));
   if (G.partials) HIPCHK(hipFree(G.partials));
+  if (G.work) HIPCHK(hipFree(G.work));
+  G.work = nullptr;
+  G.work_cap = 0;
   if (G.counter) HIPCHK(hipFree(G.counter));
   if (G.dres) HIPCHK(hipFree(G.dres));
   HIPCHK(hipStreamDestroy(G.st));
@@ -739,19 +800,8 @@ int u1_poly_volume(const u1_geom* g, int prec, const void* lat, int32_t mode, co
   if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad Polyakov mode");
   if (mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
   const RepDesc r = make_rep(g, seed, c3);
-  if (mode == U1_POLY_PLAIN) {
-    HIPCHK(u1d::launch_poly_volume(prec, lat, r, mode, 0, 0, 0.0, stream, c1, c3, (double*)P_dev, 256, G.st));
-    return U1_OK;
-  }
-  // multihit: one thread per temporal link (V-wide), then the product along t
-  double* tl = nullptr;
-  HIPCHK(hipMalloc((void**)&tl, 2 * (size_t)r.V * sizeof(double)));
-  HIPCHK(u1d::launch_temporal_links(prec, lat, r, mode, mh ? mh->proposal : 0, mh ? mh->nhit : 0,
-                                    mh ? mh->step : 0.0, stream, c1, c3, tl, 256, G.st));
-  HIPCHK(u1d::launch_tprod(tl, r, (double*)P_dev, G.st));
-  HIPCHK(hipStreamSynchronize(G.st));
-  HIPCHK(hipFree(tl));
-  return U1_OK;
+  if (mode != U1_POLY_PLAIN) RET(ensure_work(2 * (size_t)r.V));
+  return poly_volume_async(prec, lat, r, mode, mh, stream, c1, c3, (double*)P_dev, G.work);
 }
 
 int u1_temporal_links(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1_metro* mh,
@@ -790,14 +840,15 @@ int u1_poly2(const u1_geom* g, int prec, const void* lat, int32_t mode, const u1
   REQ_PTR(lat);
   REQ_PTR(out);
   RET(check_geom(g));
-  const int64_t sv = (int64_t)g->dims[0] * g->dims[1] * g->dims[2];
-  void* P = nullptr;
-  HIPCHK(hipMalloc(&P, 2 * sv * sizeof(double)));
-  int rc = u1_poly_volume(g, prec, lat, mode, mh, seed, U1_STREAM_MULTIHIT, c1, c3, P);
-  if (rc == U1_OK) rc = u1_pp(g, P, g->dims[0] / 2, out);
-  (void)hipStreamSynchronize(G.st);
-  (void)hipFree(P);
-  return rc;
+  RET(check_prec(prec));
+  if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "bad Polyakov mode");
+  if (mode == U1_POLY_MULTIHIT_STOCHASTIC) RET(check_metro(mh));
+  const RepDesc r = make_rep(g, seed, c3);
+  // work: P (2 SV doubles), then the temporal links (2 V doubles, multihit)
+  RET(ensure_work(2 * (size_t)r.SV + (mode != U1_POLY_PLAIN ? 2 * (size_t)r.V : 0)));
+  double* P = G.work;
+  RET(poly_volume_async(prec, lat, r, mode, mh, U1_STREAM_MULTIHIT, c1, c3, P, G.work + 2 * (size_t)r.SV));
+  return u1_pp(g, P, g->dims[0] / 2, out);
 }
 
 int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32_t k4, int32_t n2, int32_t k2,
@@ -969,16 +1020,16 @@ int u1_config_save(const char* path, const u1_geom* g, int prec, const void* lat
   if (!path || !lat) return fail(U1_ERR_ARG, "null argument");
   const RepDesc r = make_rep(g, 0, 0);
   const size_t n = 4 * (size_t)r.V;
-  void *no = nullptr, *d64 = nullptr;
-  HIPCHK(hipMalloc(&no, n * elem_size(prec)));
-  HIPCHK(hipMalloc(&d64, n * sizeof(double)));
-  HIPCHK(u1d::launch_convert_layout(prec, lat, no, r, 1, 256, G.st));
-  HIPCHK(u1d::launch_convert_precision(no, prec, d64, U1_F64, (int64_t)n, 256, G.st));
   std::vector<double> host(n);
-  HIPCHK(hipMemcpyAsync(host.data(), d64, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
-  HIPCHK(hipStreamSynchronize(G.st));
-  HIPCHK(hipFree(no));
-  HIPCHK(hipFree(d64));
+  {
+    DevBuf no, d64;
+    HIPCHK(no.alloc(n * elem_size(prec)));
+    HIPCHK(d64.alloc(n * sizeof(double)));
+    HIPCHK(u1d::launch_convert_layout(prec, lat, no.p, r, 1, 256, G.st));
+    HIPCHK(u1d::launch_convert_precision(no.p, prec, d64.p, U1_F64, (int64_t)n, 256, G.st));
+    HIPCHK(hipMemcpyAsync(host.data(), d64.p, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+  }
   CfgHeader h{};
   std::memcpy(h.magic, "U1AMDCFG", 8);
   h.version = 1;
@@ -1013,15 +1064,15 @@ int u1_config_load(const char* path, const u1_geom* g, int prec, void* lat, uint
   std::vector<double> host(n);
   f.read((char*)host.data(), n * sizeof(double));
   if (!f) return fail(U1_ERR_IO, std::string("truncated configuration file: ") + path);
-  void *no = nullptr, *d64 = nullptr;
-  HIPCHK(hipMalloc(&no, n * elem_size(prec)));
-  HIPCHK(hipMalloc(&d64, n * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(d64, host.data(), n * sizeof(double), hipMemcpyHostToDevice, G.st));
-  HIPCHK(u1d::launch_convert_precision(d64, U1_F64, no, prec, (int64_t)n, 256, G.st));
-  HIPCHK(u1d::launch_convert_layout(prec, no, lat, r, 0, 256, G.st));
-  HIPCHK(hipStreamSynchronize(G.st));
-  HIPCHK(hipFree(no));
-  HIPCHK(hipFree(d64));
+  {
+    DevBuf no, d64;
+    HIPCHK(no.alloc(n * elem_size(prec)));
+    HIPCHK(d64.alloc(n * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(d64.p, host.data(), n * sizeof(double), hipMemcpyHostToDevice, G.st));
+    HIPCHK(u1d::launch_convert_precision(d64.p, U1_F64, no.p, prec, (int64_t)n, 256, G.st));
+    HIPCHK(u1d::launch_convert_layout(prec, no.p, lat, r, 0, 256, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+  }
   if (seed) *seed = h.seed;
   if (sweep) *sweep = h.sweep;
   return U1_OK;
@@ -1101,6 +1152,7 @@ int u1_ensemble_create(u1_ensemble** out, int32_t nrep, const u1_geom* geoms, co
 int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
   REQ_INIT();
   if (!e) return fail(U1_ERR_ARG, "null ensemble");
+  if (nsweeps < 0) return fail(U1_ERR_ARG, "nsweeps must be >= 0");
   // tuning changes the launch layout, so it only runs before either graph exists
   if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
   if (use_graph && !e->exec) {
@@ -1108,7 +1160,10 @@ int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
     int rc = ensemble_enqueue_sweep(e);
     hipGraph_t gr = nullptr;
     hipError_t ce = hipStreamEndCapture(G.st, &gr);
-    if (rc != U1_OK) return rc;
+    if (rc != U1_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
     HIPCHK(ce);
     e->graph = gr;
     HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
@@ -1131,8 +1186,16 @@ int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series)
   if (nsweeps == 0) return U1_OK;
   if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
   if (!e->mexec) {
-    // (kept across a failed capture and reused by the next attempt)
-    if (!e->mrows) HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
+    // (kept across a failed capture and reused by the next attempt, unless a
+    // later tuning has given the update launches more workgroups)
+    if (e->mrows && e->mrows_blocks < e->upd_blocks) {
+      HIPCHK(hipFree(e->mrows));
+      e->mrows = nullptr;
+    }
+    if (!e->mrows) {
+      HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
+      e->mrows_blocks = e->upd_blocks;
+    }
     if (!e->series_dev)
       HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
     if (!e->base_dev) HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
@@ -1259,7 +1322,7 @@ int u1_tune_load(const char* path) {
       return fail(U1_ERR_IO, std::string("Cache file ") + path + " does not match this build");
   }
   std::getline(f, line);  // column description
-  int n = 0;
+  std::vector<std::pair<std::string, Cfg>> parsed;  // applied only if every line is valid
   while (std::getline(f, line)) {
     if (line.empty()) continue;
     std::istringstream ls(line);
@@ -1268,10 +1331,17 @@ int u1_tune_load(const char* path) {
     if (!(std::getline(ls, vol, '\t') && std::getline(ls, name, '\t') && std::getline(ls, aux, '\t')))
       return fail(U1_ERR_IO, std::string("Bad format in ") + path);
     if (!(ls >> bx >> by >> bz >> gx >> gy >> gz >> sh)) return fail(U1_ERR_IO, std::string("Bad format in ") + path);
-    G.tunecache[vol + "\t" + name] = Cfg{bx, (by == 2 || by == 4) ? by : 1, bz == 2 ? 1 : 0};
-    ++n;
+    // only the shapes the tuner itself tries are accepted: the kernels assume
+    // whole wavefronts, at most 512 threads and 4 sites per thread (binary32)
+    const bool f32 = name.size() > 4 && name.compare(name.size() - 4, 4, "_f32") == 0;
+    bool known = false;
+    for (const Cfg& c : kShapes) known = known || (c.block == bx && c.spt == by);
+    if (!known || (by == 4 && !f32) || (bz != 1 && bz != 2))
+      return fail(U1_ERR_IO, std::string("Bad launch shape in ") + path + ": " + line);
+    parsed.push_back({vol + "\t" + name, Cfg{bx, by, bz == 2 ? 1 : 0}});
   }
-  G.tune_loaded = n;
+  for (const auto& kv : parsed) G.tunecache[kv.first] = kv.second;
+  G.tune_loaded = (int)parsed.size();
   return U1_OK;
 }
 
@@ -1311,8 +1381,9 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   if (nsweeps < 1) return fail(U1_ERR_ARG, "nsweeps must be >= 1");
   if (measure_plaq < 0 || measure_plaq > 2) return fail(U1_ERR_ARG, "measure_plaq must be 0, 1 or 2");
   if (measure_plaq == 1 && metrop + ovrn == 0) return fail(U1_ERR_ARG, "no measuring pass");
-  double* series = nullptr;
-  if (measure_plaq) HIPCHK(hipMalloc((void**)&series, 4 * (size_t)nsweeps * sizeof(double)));
+  DevBuf series_buf;
+  if (measure_plaq) HIPCHK(series_buf.alloc(4 * (size_t)nsweeps * sizeof(double)));
+  double* series = series_buf.as<double>();
   {  // tune outside the timed region
     PhaseArgs a{};
     a.single = make_rep(g, seed, 0);
@@ -1325,16 +1396,17 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   }
   // Timed region: every sweep brackets its Metropolis and OR launch groups
   // with events (kernel_ms), plus one event pair around the whole region.
+  EventSet es;
   std::vector<GroupEvents> gev(kernel_ms ? (size_t)nsweeps : 0);
   for (auto& e : gev) {
-    HIPCHK(hipEventCreate(&e.start));
-    HIPCHK(hipEventCreate(&e.after_metro));
-    HIPCHK(hipEventCreate(&e.before_meas));
-    HIPCHK(hipEventCreate(&e.after_or));
+    HIPCHK(es.create(&e.start));
+    HIPCHK(es.create(&e.after_metro));
+    HIPCHK(es.create(&e.before_meas));
+    HIPCHK(es.create(&e.after_or));
   }
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
   HIPCHK(hipStreamSynchronize(G.st));
   HIPCHK(hipEventRecord(e0, G.st));
   for (int s = 0; s < nsweeps; ++s) {
@@ -1361,10 +1433,6 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
       tm += a;
       to += b;
       tq += c;
-      HIPCHK(hipEventDestroy(e.start));
-      HIPCHK(hipEventDestroy(e.after_metro));
-      HIPCHK(hipEventDestroy(e.before_meas));
-      HIPCHK(hipEventDestroy(e.after_or));
     }
     kernel_ms[0] = metrop > 0 ? tm / (8.0 * metrop * nsweeps) : 0.0;
     kernel_ms[1] = n_or_plain > 0 ? to / ((double)n_or_plain * nsweeps) : 0.0;
@@ -1372,9 +1440,6 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   }
   if (measure_plaq && plaq_series)
     HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));
-  if (series) HIPCHK(hipFree(series));
-  HIPCHK(hipEventDestroy(e0));
-  HIPCHK(hipEventDestroy(e1));
   return U1_OK;
 }
 
@@ -1384,18 +1449,15 @@ int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, 
   if (fn < 0 || fn > 8 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
   if (n == 0) return U1_OK;
   const int nout = (fn == 0 || fn == 3 || fn == 5 || fn == 7) ? 2 : 1;
-  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
-  HIPCHK(hipMalloc((void**)&dx, n * sizeof(double)));
-  HIPCHK(hipMalloc((void**)&dy, n * sizeof(double)));
-  HIPCHK(hipMalloc((void**)&dout, nout * n * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
-  HIPCHK(hipMemcpyAsync(dy, y ? y : x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
-  HIPCHK(u1d::launch_selftest_math(fn, dx, dy, dout, n, G.st));
-  HIPCHK(hipMemcpyAsync(out, dout, nout * n * sizeof(double), hipMemcpyDeviceToHost, G.st));
+  DevBuf dx, dy, dout;
+  HIPCHK(dx.alloc(n * sizeof(double)));
+  HIPCHK(dy.alloc(n * sizeof(double)));
+  HIPCHK(dout.alloc(nout * n * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(dy.p, y ? y : x, n * sizeof(double), hipMemcpyHostToDevice, G.st));
+  HIPCHK(u1d::launch_selftest_math(fn, dx.as<double>(), dy.as<double>(), dout.as<double>(), n, G.st));
+  HIPCHK(hipMemcpyAsync(out, dout.p, nout * n * sizeof(double), hipMemcpyDeviceToHost, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
-  HIPCHK(hipFree(dx));
-  HIPCHK(hipFree(dy));
-  HIPCHK(hipFree(dout));
   return U1_OK;
 }
 
@@ -1439,18 +1501,15 @@ int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, 
   REQ_INIT();
   if (!ctr || !key || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
   if (n == 0) return U1_OK;
-  uint32_t *dc = nullptr, *dk = nullptr, *dout = nullptr;
-  HIPCHK(hipMalloc((void**)&dc, 4 * n * sizeof(uint32_t)));
-  HIPCHK(hipMalloc((void**)&dk, 2 * n * sizeof(uint32_t)));
-  HIPCHK(hipMalloc((void**)&dout, 4 * n * sizeof(uint32_t)));
-  HIPCHK(hipMemcpyAsync(dc, ctr, 4 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
-  HIPCHK(hipMemcpyAsync(dk, key, 2 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
-  HIPCHK(u1d::launch_selftest_philox(dc, dk, dout, n, G.st));
-  HIPCHK(hipMemcpyAsync(out, dout, 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, G.st));
+  DevBuf dc, dk, dout;
+  HIPCHK(dc.alloc(4 * n * sizeof(uint32_t)));
+  HIPCHK(dk.alloc(2 * n * sizeof(uint32_t)));
+  HIPCHK(dout.alloc(4 * n * sizeof(uint32_t)));
+  HIPCHK(hipMemcpyAsync(dc.p, ctr, 4 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(dk.p, key, 2 * n * sizeof(uint32_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(u1d::launch_selftest_philox(dc.as<uint32_t>(), dk.as<uint32_t>(), dout.as<uint32_t>(), n, G.st));
+  HIPCHK(hipMemcpyAsync(out, dout.p, 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
-  HIPCHK(hipFree(dc));
-  HIPCHK(hipFree(dk));
-  HIPCHK(hipFree(dout));
   return U1_OK;
 }
 

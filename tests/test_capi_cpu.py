@@ -74,6 +74,27 @@ def test_tune_cache_roundtrip(tmp_path):
         u1.tune_load(str(stale))
 
 
+@pytest.mark.parametrize("shape", ["100\t1\t1", "0\t1\t1", "1024\t1\t1", "128\t3\t1", "128\t1\t3", "64\t4\t2"])
+def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, shape):
+    """A cache entry must be one of the tuner's own shapes (whole wavefronts,
+    <= 512 threads, 1/2/4 sites per thread, 4 only for binary32), and a file
+    with one bad line changes nothing."""
+    import latticeu1_amd as u1
+
+    lib = u1.load()
+    head = "tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n"
+    good = "6x6x6x6\tOverRelaxation_f32\tthreads=halfvolume\t256\t4\t2\t0\t1\t1\t0\t# x\n"
+    n0 = C.c_int32(0)
+    assert lib.u1_tune_entries(C.byref(n0)) == 0
+    p = tmp_path / "bad_shape.csv"
+    p.write_text(head + good + f"6x6x6x6\tMetropolis_f64\tthreads=halfvolume\t{shape}\t0\t1\t1\t0\t# x\n")
+    with pytest.raises(u1.U1Error) as e:
+        u1.tune_load(str(p))
+    assert e.value.code == 5 and "launch shape" in str(e.value)
+    n1 = C.c_int32(0)
+    assert lib.u1_tune_entries(C.byref(n1)) == 0 and n1.value == n0.value
+
+
 def test_config_file_validation_on_host(tmp_path):
     """u1_config_load checks the header before touching the device."""
     import struct
