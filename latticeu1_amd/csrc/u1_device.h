@@ -105,7 +105,8 @@ __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
 // covers chunk (b % bpp) of the (z, t) plane b / bpp; thread tid of a
 // workgroup of bdim threads takes even/odd site id = plane * P + chunk * bdim
 // + tid, P = Nx/2 * Ny.  z, t (and so the z/t neighbour offsets) are uniform
-// across the workgroup.  valid is false for the tail threads of a plane.
+// across the workgroup.  valid is false for the tail threads of a plane,
+// which are given the plane's last site.
 struct PhaseSite {
   bool valid;
   uint32_t id, t;
@@ -120,8 +121,11 @@ __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b
   const uint32_t t = mdiv(plane, r.div_nz_m, r.div_nz_s);
   const int32_t z = (int32_t)(plane - t * (uint32_t)r.N[2]);
   const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
-  const uint32_t i = chunk * bdim + tid;
-  s.valid = i < P;
+  const uint32_t i_raw = chunk * bdim + tid;
+  s.valid = i_raw < P;
+  // a thread past the end of the plane takes the plane's last site: its loads
+  // stay in bounds (and hit the cache lines of that site); it stores nothing
+  const uint32_t i = s.valid ? i_raw : P - 1;
   s.t = t;
   s.z = z;
   const uint32_t y = mdiv(i, r.div_nxh_m, r.div_nxh_s);

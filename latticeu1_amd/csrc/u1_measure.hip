@@ -84,13 +84,7 @@ __device__ C64 temporal_value(const Real* __restrict__ lat, const RepDesc& r, in
   Real re_old = Sre * c0 - Sim * s0;
   const Real beta = (Real)r.beta;
   double ac = 0.0, as = 0.0;
-  for (int h = 0; h < nhit; ++h) {
-    uint32_t wu, wb;
-    {
-      const u1r::U32x4 w = u1r::draw(r.seed, (uint32_t)l, c1, u1r::ctr2(stream, 0, (uint32_t)(h >> 1)), c3);
-      wu = (h & 1) ? w.v[2] : w.v[0];
-      wb = (h & 1) ? w.v[3] : w.v[1];
-    }
+  auto hit = [&](uint32_t wu, uint32_t wb) {
     const double u = u1r::u01_d(wu);
     const Real tn = (Real)(proposal == 0 ? u1m::K_PI * (2.0 * u - 1.0) : u1m::wrap_d((double)th + step * (2.0 * u - 1.0)));
     Real sn, cn;
@@ -105,6 +99,12 @@ __device__ C64 temporal_value(const Real* __restrict__ lat, const RepDesc& r, in
     }
     ac += (double)c0;
     as += (double)s0;
+  };
+  // hits 2j and 2j + 1 read words (0, 1) and (2, 3) of one Philox call
+  for (int h = 0; h < nhit; h += 2) {
+    const u1r::U32x4 w = u1r::draw(r.seed, (uint32_t)l, c1, u1r::ctr2(stream, 0, (uint32_t)(h >> 1)), c3);
+    hit(w.v[0], w.v[1]);
+    if (h + 1 < nhit) hit(w.v[2], w.v[3]);
   }
   return {ac / (double)nhit, as / (double)nhit};
 }
@@ -132,14 +132,7 @@ __global__ __launch_bounds__(256) void k_plaquette(const Real* __restrict__ lat_
   if (parity) b -= half;
   PhaseSite ps[SPT];
 #pragma unroll
-  for (int k = 0; k < SPT; ++k) {
-    phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, parity, ps[k]);
-    if (!ps[k].valid) {
-      ps[k].id = 0;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) ps[k].fwd[d] = ps[k].bwd[d] = 0;
-    }
-  }
+  for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, parity, ps[k]);
   Real tm[SPT][4], tf[SPT][4][4];  // tf[k][nu][mu] = theta_nu(x + mu), mu < nu or mu > nu
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {

@@ -42,21 +42,42 @@ static constexpr uint32_t PHILOX_M1 = 0xCD9E8D57u;
 static constexpr uint32_t PHILOX_W0 = 0x9E3779B9u;  // golden ratio
 static constexpr uint32_t PHILOX_W1 = 0xBB67AE85u;  // sqrt(3) - 1
 
+// a ^ b ^ c.  On gfx950 one v_bitop3_b32 (truth table 0x96) instead of two
+// v_xor_b32; the compiler does not form it from the XOR chain by itself.
+U1_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
+// One Philox4x32 round with round key (k0, k1).  FUSED selects xor3 for the
+// two three-way XORs.
+template <bool FUSED>
+U1_HD U32x4 philox_round(U32x4 c, uint32_t k0, uint32_t k1) {
+  const uint64_t p0 = (uint64_t)PHILOX_M0 * c.v[0];
+  const uint64_t p1 = (uint64_t)PHILOX_M1 * c.v[2];
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  U32x4 n;
+  n.v[0] = FUSED ? xor3(hi1, c.v[1], k0) : hi1 ^ c.v[1] ^ k0;
+  n.v[1] = lo1;
+  n.v[2] = FUSED ? xor3(hi0, c.v[3], k1) : hi0 ^ c.v[3] ^ k1;
+  n.v[3] = lo0;
+  return n;
+}
+
 U1_HD U32x4 philox4x32_10(U32x4 c, uint32_t k0, uint32_t k1) {
+  // Round 1 keeps plain XORs: with the link index in c0 and wavefront-uniform
+  // c1..c3 and key, its first output word is uniform and stays in scalar
+  // registers (a vector bitop3 would move it to the vector unit).
+  c = philox_round<false>(c, k0, k1);
 #pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint64_t p0 = (uint64_t)PHILOX_M0 * c.v[0];
-    const uint64_t p1 = (uint64_t)PHILOX_M1 * c.v[2];
-    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
-    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    U32x4 n;
-    n.v[0] = hi1 ^ c.v[1] ^ k0;
-    n.v[1] = lo1;
-    n.v[2] = hi0 ^ c.v[3] ^ k1;
-    n.v[3] = lo0;
-    c = n;
+  for (int r = 1; r < 10; ++r) {
     k0 += PHILOX_W0;
     k1 += PHILOX_W1;
+    c = philox_round<true>(c, k0, k1);
   }
   return c;
 }

@@ -19,10 +19,11 @@
 // with the same u1_math.h functions.  The library is compiled with
 // -ffp-contract=off so that no FMA is formed implicitly.
 //
-// FP32-bulk variant: angles, staples, the Metropolis weight and the OR angle
-// in binary32 with the hardware transcendentals (v_sin_f32, v_cos_f32,
-// v_exp_f32, v_rcp_f32); proposals and the wrap into [-pi, pi) in binary64
-// before the single rounding to binary32.  Its parity bar is statistical.
+// FP32-bulk variant: angles, staples, proposals, the Metropolis weight and the
+// OR angle in binary32 with the hardware transcendentals (v_sin_f32,
+// v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs; the OR wrap in binary64
+// before the single rounding to binary32.  It reads the same Philox words as
+// the FP64 path.  Its parity bar is statistical.
 #include "u1_kernels.h"
 
 namespace u1d {
@@ -110,9 +111,9 @@ struct MetroState {
 };
 
 // One Metropolis hit with a fixed staple (lattice_functions.h:68-80 in the
-// Re(S e^{i theta}) form).  wu: proposal word, wb: accept-test word.  The
-// proposal is formed in FP64 for both precisions; for theta + delta one wrap
-// step each way suffices because |delta| <= step <= pi.
+// Re(S e^{i theta}) form), binary64: the oracle's operations in its order.
+// wu: proposal word, wb: accept-test word.  For theta + delta one wrap step
+// each way suffices because |delta| <= step <= pi.
 template <typename Real>
 __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, Real beta, int proposal, double step,
                                           uint32_t wu, uint32_t wb) {
@@ -140,10 +141,36 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
   }
 }
 
+// The same hit in binary32, from the same two words.  The proposal is
+// a (2 wu 2^-32 - 1) as one FMA, with a = pi (independent: pscale = pi 2^-31,
+// poff = pi) or a = step (theta + delta); theta + delta is brought back by the
+// nearest multiple of 2 pi (|theta + delta| <= 2 pi), landing in [-pi, pi] up
+// to binary32 rounding.  The accept test exp(beta dRe) > wb 2^-32 is evaluated
+// as 2^(beta log2(e) dRe + 32) > wb (bl2e = beta log2(e)): one v_exp_f32 and
+// the word converted without scaling.
+__device__ __forceinline__ void metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, int proposal,
+                                              float pscale, float poff, uint32_t wu, uint32_t wb) {
+  float tn = __builtin_fmaf((float)wu, pscale, -poff);
+  if (proposal != 0) {
+    tn = m.th + tn;
+    tn = __builtin_fmaf(-__builtin_rintf(tn * 0x1.45f306p-3f), u1m::KF_TWO_PI, tn);  // 1 / (2 pi)
+  }
+  float sn, cn;
+  UMath<float>::sincos(tn, &sn, &cn);
+  const float re_new = __builtin_fmaf(S.re, cn, -(S.im * sn));
+  const float x = __builtin_fmaf(bl2e, re_new - m.re_old, 32.0f);
+  if (__builtin_amdgcn_exp2f(x) > (float)wb) {
+    m.th = tn;
+    m.re_old = re_new;
+    ++m.acc;
+  }
+}
+
 // SPT even/odd sites per thread: sites chunk*SPT*bdim + k*bdim + tid, k < SPT,
 // of one (z, t) plane.  All angle loads of all SPT sites are issued before the
 // transcendental work, so each wave keeps SPT x 19 loads in flight.  Threads
-// past the end of a plane are pointed at site 0 (in bounds) and do not store.
+// past the end of a plane work on the plane's last site (in bounds) and do not
+// store.
 //
 // MEAS (MU >= 1, parity 1 of the last pass of a sweep): the logical
 // workgroup L also writes row L of a.meas with its sums over the plaquettes of
@@ -171,11 +198,6 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   PlaqPieces<Real> pp[SPT];
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {
-    if (!ps[k].valid) {
-      ps[k].id = 0;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) ps[k].fwd[d] = ps[k].bwd[d] = 0;
-    }
     same[k] = (uint32_t)(a.parity * r.HV) + ps[k].id;
     const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
     pp[k] = {0, 0, 0, 0};
@@ -192,19 +214,31 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
       m[k].th = th[k];
       Real s0, c0;
       UMath<Real>::sincos(th[k], &s0, &c0);
-      m[k].re_old = S[k].re * c0 - S[k].im * s0;
+      if constexpr (sizeof(Real) == sizeof(float))
+        m[k].re_old = __builtin_fmaf(S[k].re, c0, -(S[k].im * s0));
+      else
+        m[k].re_old = S[k].re * c0 - S[k].im * s0;
       m[k].acc = 0;
       l[k] = same[k] + (uint32_t)MU * (uint32_t)r.V;
     }
     const uint32_t sweep = a.sweep_dev ? *a.sweep_dev * a.sweep_mul + a.sweep : a.sweep;
     const Real beta = (Real)r.beta;
+    // binary32 hit constants (metro_hit_f32)
+    const float bl2e = (float)(r.beta * 1.4426950408889634);
+    const float amp = a.proposal == 0 ? u1m::KF_PI : (float)a.step;
+    const float pscale = amp * 0x1p-31f;
     for (int h = 0; h < a.nhit; h += 2) {
       const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
-        metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
-        if (h + 1 < a.nhit) metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+        if constexpr (sizeof(Real) == sizeof(float)) {
+          metro_hit_f32(m[k], S[k], bl2e, a.proposal, pscale, amp, w.v[0], w.v[1]);
+          if (h + 1 < a.nhit) metro_hit_f32(m[k], S[k], bl2e, a.proposal, pscale, amp, w.v[2], w.v[3]);
+        } else {
+          metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
+          if (h + 1 < a.nhit) metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+        }
       }
     }
 #pragma unroll

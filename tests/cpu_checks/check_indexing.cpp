@@ -97,7 +97,19 @@ int main() {
           for (int tid = 0; tid < B; ++tid) {
             u1d::PhaseSite ps;
             u1d::phase_site(r, blk, tid, B, parity, ps);
-            if (!ps.valid) continue;
+            if (!ps.valid) {
+              // tail threads load (and never store): every address in bounds
+              for (int d = 0; d < 4; ++d) {
+                const long f = (long)ps.id + ps.fwd[d], bk = (long)ps.id + ps.bwd[d];
+                const long ff = f + ps.fwd[(d + 1) % 4], fb = f + ps.bwd[(d + 1) % 4];
+                if (ps.id >= (uint32_t)r.HV || f < 0 || f >= r.HV || bk < 0 || bk >= r.HV || ff < 0 ||
+                    ff >= r.HV || fb < 0 || fb >= r.HV) {
+                  printf("phase_site: tail thread out of bounds\n");
+                  return 1;
+                }
+              }
+              continue;
+            }
             if (ps.id >= (uint32_t)r.HV || seen[ps.id]) { printf("phase_site: bad/duplicate id %u\n", ps.id); return 1; }
             seen[ps.id] = 1;
             // brute-force coordinates of (id, parity): indexEO restated

@@ -350,6 +350,12 @@ def test_geometry_and_argument_limits(u1):
     with pytest.raises(u1.U1Error):
         lat.update_plaquette(0, 0, u1.BENCH_METRO, 1, 0)
     assert np.all(lat.to_host() == 0.0)  # nothing ran
+    e = u1.Ensemble([ok], [5], [0], u1.F32, 1, 3, u1.BENCH_METRO)
+    for bad in (lambda: e.sweep(-1), lambda: u1.call("u1_ensemble_sweep_plaquette", e.ptr, -2, u1._dp(out))):
+        with pytest.raises(u1.U1Error):
+            bad()
+    assert e.info()[0] == 0 and np.all(e.read(0) == 0.0)
+    e.destroy()
 
 
 def test_layout_and_fields_fp64(u1, orc, thermal):
@@ -397,6 +403,39 @@ def test_fp32_overrelaxation_tracks_fp64(u1, orc):
     a32.update(0, 1, u1.BENCH_METRO, 3, 0)
     d = np.angle(np.exp(1j * (a64.to_host() - a32.to_host().astype(np.float64))))
     assert np.median(np.abs(d)) < 1e-5
+
+
+@pytest.mark.parametrize("proposal,nhit,step,beta", [(0, 1, 0.0, 1.0), (0, 2, 0.0, 0.7), (1, 4, 0.5, 1.0),
+                                                     (1, 3, 2.5, 1.3)])
+def test_fp32_metropolis_tracks_fp64(u1, orc, proposal, nhit, step, beta):
+    """The binary32 hit (proposal FMA, wrap, 2^(beta log2(e) dRe + 24) > u24)
+    reads the same Philox words as the binary64 hit, which matches the oracle
+    bit for bit.  From a configuration exactly representable in FP32, one
+    Metropolis pass in each precision: the accept decisions agree except for
+    the rare draws within rounding of the threshold (and the neighbours they
+    perturb later in the pass), and the angles agree to FP32 rounding."""
+    dims = (8, 8, 8, 8)
+    g = u1.geom(dims, beta)
+    og = orc.geom(dims, beta)
+    ref = orc.hotstart(og, 17)
+    for s in range(5):
+        orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 17, s)
+    start = ref.astype(np.float32).astype(np.float64)
+    m = u1.metro(proposal, nhit, step)
+    a64 = u1.Lattice.from_host(g, start, u1.F64)
+    a32 = u1.Lattice.from_host(g, start, u1.F32)
+    n64 = a64.update(1, 0, m, 23, 7, count_accepts=True)
+    n32 = a32.update(1, 0, m, 23, 7, count_accepts=True)
+    d = np.abs(np.angle(np.exp(1j * (a64.to_host() - a32.to_host().astype(np.float64)))))
+    links = d.size
+    moved = np.mean(np.abs(np.angle(np.exp(1j * (a64.to_host() - start)))) > 1e-6)
+    say(f"fp32 Metropolis vs fp64: proposal {proposal} nhit {nhit} step {step} beta {beta}: accepts "
+        f"{n32} vs {n64} of {links * nhit} hits, links moved {moved:.3f}, links off by > 1e-4: {np.sum(d > 1e-4)}, "
+        f"median |d| {np.median(d):.1e}")
+    assert moved > 0.2
+    assert abs(n32 - n64) <= 2e-3 * links * nhit
+    assert np.sum(d > 1e-4) <= 2e-3 * links
+    assert np.median(d) < 1e-6
 
 
 def test_fp32_chain_statistics_match_fp64(u1):

@@ -3,7 +3,8 @@
 Errors come from the spread of independent replicas (different seeds), not
 from a single chain's jackknife, so long autocorrelations cannot hide a bias.
 beta = 0.8 (confined) and 1.3 (Coulomb phase) are far from the U(1) phase
-transition near beta = 1.011, where single chains are metastable.
+transition near beta = 1.011, where single chains are metastable; beta = 1.0
+and 0.95 sit next to it, with many replicas instead of long chains.
 """
 import math
 import sys
@@ -76,3 +77,27 @@ def test_fp32_fp64_independent_chains_48x4(u1):
     z = d / math.hypot(res["fp32"][1], res["fp64"][1])
     say(f"48^4 <P> FP32 - FP64 over independent chains: {d:+.2e} = {z:+.2f} combined sigma")
     assert abs(z) < 3.0
+
+
+@pytest.mark.parametrize("dims,nrep,beta", [((12, 12, 12, 12), 128, 1.0), ((16, 16, 16, 16), 128, 1.0),
+                                            ((24, 24, 24, 24), 64, 1.0), ((16, 16, 16, 16), 128, 0.95)])
+def test_fp32_fp64_ensemble_means_near_transition(u1, dims, nrep, beta):
+    """Near the transition, where <P> moves fastest with beta (a shift of the
+    effective coupling would show most): independent replicas through the
+    batched path, 400 thermalisation sweeps, then 1200 sweeps with the
+    plaquette summed inside every sweep."""
+    def means(prec, seed0):
+        geoms = [u1.geom(dims, beta) for _ in range(nrep)]
+        e = u1.Ensemble(geoms, [seed0 + 7919 * i for i in range(nrep)], [1] * nrep, prec, 1, 3, u1.BENCH_METRO)
+        e.sweep(400, graph=True)
+        s = e.sweep_plaquette(1200)
+        e.destroy()
+        return (1.0 - 0.5 * (s[:, :, 0] + s[:, :, 2])).mean(axis=0)  # mean_plaquette per sweep and replica
+
+    m32, m64 = means(u1.F32, 1000), means(u1.F64, 5000)
+    a, ea = m32.mean(), m32.std(ddof=1) / math.sqrt(nrep)
+    b, eb = m64.mean(), m64.std(ddof=1) / math.sqrt(nrep)
+    z = (a - b) / math.hypot(ea, eb)
+    say(f"{dims[0]}^4 x {nrep} beta={beta}: <P> fp32 {a:.7f} +- {ea:.1e}, fp64 {b:.7f} +- {eb:.1e}, "
+        f"diff {a - b:+.2e} = {z:+.2f} sigma")
+    assert abs(z) < 3.5
