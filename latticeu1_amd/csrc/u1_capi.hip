@@ -155,6 +155,7 @@ RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
   r.div_nz_s = c.s;
   r.beta = g->beta;
   r.aniso = g->aniso;
+  r.inv_aniso_f = (float)(1.0 / g->aniso);
   r.seed = seed;
   r.lat_off = 0;
   return r;

@@ -35,7 +35,7 @@ struct RepDesc {
   uint32_t div_nxh_s, div_ny_s, div_nz_s;  // and their shifts
   int32_t bpp;                             // update launches: workgroups per (z, t) plane
   uint32_t div_bpp_m, div_bpp_s;           // magic division by bpp
-  uint32_t pad0;
+  float inv_aniso_f;                       // binary32 1 / aniso (binary32 space-space staple weight)
   double beta;
   double aniso;
   uint64_t seed;

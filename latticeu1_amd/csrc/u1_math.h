@@ -143,16 +143,13 @@ U1_HD double wrap_d(double x) {
   return x;
 }
 
-// Shorter wrap for |x| <= 3 pi + 1e-6 (the binary32 overrelaxation angle):
-// one rint reduction by 2 pi, then one correction each way.  Lands in
-// [-pi, pi) and equals wrap_d within 1 ulp of binary64 (checked on the CPU);
-// the binary64 kernels keep wrap_d, which the oracle uses.
-U1_HD double wrap3_d(double x) {
-  const double k = rintd(x * (K_TWO_OVER_PI * 0.25));
-  double y = fmad(-k, K_TWO_PI, x);
-  if (y >= K_PI) y -= K_TWO_PI;
-  if (y < -K_PI) y += K_TWO_PI;
-  return y;
+// Binary32 wrap for the binary32 update kernels: x - k 2pi with
+// k = rint(x / 2pi), 2pi split into two binary32 parts (Cody-Waite), so the
+// reduction itself adds at most about one ulp.  The result lies in [-pi, pi]
+// up to binary32 rounding (checked on the CPU against the binary64 wrap_d).
+U1_HD float wrap_f(float x) {
+  const float k = __builtin_rintf(x * KF_INV_TWO_PI);
+  return __builtin_fmaf(-k, KF_TWO_PI_LO, __builtin_fmaf(-k, KF_TWO_PI, x));
 }
 
 // ------------------------------------------------------- Bessel I0, I1 (FP64)

@@ -19,11 +19,10 @@
 // with the same u1_math.h functions.  The library is compiled with
 // -ffp-contract=off so that no FMA is formed implicitly.
 //
-// FP32-bulk variant: angles, staples, proposals, the Metropolis weight and the
-// OR angle in binary32 with the hardware transcendentals (v_sin_f32,
-// v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs; the OR wrap in binary64
-// before the single rounding to binary32.  It reads the same Philox words as
-// the FP64 path.  Its parity bar is statistical.
+// FP32-bulk variant: angles, staples, proposals, the Metropolis weight, the OR
+// angle and the wraps in binary32 with the hardware transcendentals
+// (v_sin_f32, v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs.  It reads
+// the same Philox words as the FP64 path.  Its parity bar is statistical.
 #include "u1_kernels.h"
 
 namespace u1d {
@@ -144,17 +143,14 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // The same hit in binary32, from the same two words.  The proposal is
 // a (2 wu 2^-32 - 1) as one FMA, with a = pi (independent: pscale = pi 2^-31,
 // poff = pi) or a = step (theta + delta); theta + delta is brought back by the
-// nearest multiple of 2 pi (|theta + delta| <= 2 pi), landing in [-pi, pi] up
-// to binary32 rounding.  The accept test exp(beta dRe) > wb 2^-32 is evaluated
+// nearest multiple of 2 pi (wrap_f), landing in [-pi, pi] up to binary32
+// rounding.  The accept test exp(beta dRe) > wb 2^-32 is evaluated
 // as 2^(beta log2(e) dRe + 32) > wb (bl2e = beta log2(e)): one v_exp_f32 and
 // the word converted without scaling.
 __device__ __forceinline__ void metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, int proposal,
                                               float pscale, float poff, uint32_t wu, uint32_t wb) {
   float tn = __builtin_fmaf((float)wu, pscale, -poff);
-  if (proposal != 0) {
-    tn = m.th + tn;
-    tn = __builtin_fmaf(-__builtin_rintf(tn * 0x1.45f306p-3f), u1m::KF_TWO_PI, tn);  // 1 / (2 pi)
-  }
+  if (proposal != 0) tn = u1m::wrap_f(m.th + tn);
   float sn, cn;
   UMath<float>::sincos(tn, &sn, &cn);
   const float re_new = __builtin_fmaf(S.re, cn, -(S.im * sn));
@@ -202,7 +198,7 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
     pp[k] = {0, 0, 0, 0};
     S[k] = staple_at<Real, MU, MEAS>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, (Real)r.aniso,
-                                     (Real)(1.0 / r.aniso), aniso, &pp[k]);
+                                     (Real)r.inv_aniso_f, aniso, &pp[k]);
     th[k] = ld(pmu, same[k]);
   }
   int acc = 0;
@@ -249,10 +245,12 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   } else {
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
-      // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped in FP64
+      // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped
       const Real t2 = UMath<Real>::atan2(S[k].im, S[k].re);
-      const double x = -(double)th[k] - 2.0 * (double)t2;
-      th[k] = (Real)(sizeof(Real) == sizeof(double) ? u1m::wrap_d(x) : u1m::wrap3_d(x));
+      if constexpr (sizeof(Real) == sizeof(float))
+        th[k] = u1m::wrap_f(__builtin_fmaf(-2.0f, t2, -th[k]));
+      else
+        th[k] = u1m::wrap_d(-th[k] - 2.0 * t2);
     }
   }
 #pragma unroll
