@@ -142,15 +142,16 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 
 // The same hit in binary32, from the same two words.  The proposal is
 // a (2 wu 2^-32 - 1) as one FMA, with a = pi (independent: pscale = pi 2^-31,
-// poff = pi) or a = step (theta + delta); theta + delta is brought back by the
-// nearest multiple of 2 pi (wrap_f), landing in [-pi, pi] up to binary32
-// rounding.  The accept test exp(beta dRe) > wb 2^-32 is evaluated
-// as 2^(beta log2(e) dRe + 32) > wb (bl2e = beta log2(e)): one v_exp_f32 and
-// the word converted without scaling.
-__device__ __forceinline__ void metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, int proposal,
+// poff = pi) or a = step (theta + delta).  theta + delta is not wrapped here:
+// the caller wraps theta after each pair of hits (|theta| <= 3 pi in between,
+// well inside the hardware sin/cos range).  The accept test
+// exp(beta dRe) > wb 2^-32 is evaluated as 2^(beta log2(e) dRe + 32) > wb
+// (bl2e = beta log2(e)): one v_exp_f32 and the word converted without scaling.
+// thm is 1 for theta + delta and 0 for the independent proposal, so both
+// forms are one more FMA (theta * thm + offset, rounded once).
+__device__ __forceinline__ void metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, float thm,
                                               float pscale, float poff, uint32_t wu, uint32_t wb) {
-  float tn = __builtin_fmaf((float)wu, pscale, -poff);
-  if (proposal != 0) tn = u1m::wrap_f(m.th + tn);
+  const float tn = __builtin_fmaf(m.th, thm, __builtin_fmaf((float)wu, pscale, -poff));
   float sn, cn;
   UMath<float>::sincos(tn, &sn, &cn);
   const float re_new = __builtin_fmaf(S.re, cn, -(S.im * sn));
@@ -223,14 +224,16 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     const float bl2e = (float)(r.beta * 1.4426950408889634);
     const float amp = a.proposal == 0 ? u1m::KF_PI : (float)a.step;
     const float pscale = amp * 0x1p-31f;
+    const float thm = a.proposal == 0 ? 0.0f : 1.0f;
     for (int h = 0; h < a.nhit; h += 2) {
       const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
         if constexpr (sizeof(Real) == sizeof(float)) {
-          metro_hit_f32(m[k], S[k], bl2e, a.proposal, pscale, amp, w.v[0], w.v[1]);
-          if (h + 1 < a.nhit) metro_hit_f32(m[k], S[k], bl2e, a.proposal, pscale, amp, w.v[2], w.v[3]);
+          metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[0], w.v[1]);
+          if (h + 1 < a.nhit) metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[2], w.v[3]);
+          m[k].th = u1m::wrap_f(m[k].th);  // back into [-pi, pi] once per pair
         } else {
           metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
           if (h + 1 < a.nhit) metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
