@@ -254,11 +254,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   v += dpp_f64<0x140>(v);
   return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
 }
-// Integer sum over the wave (accept counters; order-independent).
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+
+// Wavefront lane mask of a predicate (wave-uniform, in scalar registers), and
+// the number of lanes of mask m for which p holds: the population count of
+// the compare mask on the scalar unit.
+__device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ uint32_t count_wave(bool p, uint64_t m) {
+  return (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(p) & m);
 }
 
 // Block sum of NV doubles per thread into out[0..NV) (valid in thread 0).

@@ -106,7 +106,6 @@ __device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, 
 template <typename Real>
 struct MetroState {
   Real th, re_old;
-  int acc;
 };
 
 // One Metropolis hit with a fixed staple (lattice_functions.h:68-80 in the
@@ -114,7 +113,7 @@ struct MetroState {
 // wu: proposal word, wb: accept-test word.  For theta + delta one wrap step
 // each way suffices because |delta| <= step <= pi.
 template <typename Real>
-__device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, Real beta, int proposal, double step,
+__device__ __forceinline__ bool metro_hit(MetroState<Real>& m, Staple<Real> S, Real beta, int proposal, double step,
                                           uint32_t wu, uint32_t wb) {
   // a * (2u - 1) with u = wu 2^-32 as one FMA: 2u - 1 is exact in binary64,
   // so fma(wu, a 2^-31, -a) is the same correctly rounded product as the
@@ -133,11 +132,12 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
   const Real re_new = S.re * cn - S.im * sn;
   const Real dS = UMath<Real>::exp(beta * (re_new - m.re_old));
   const Real b = Prec<Real>::u01(wb);
-  if (dS > b) {
+  const bool ok = dS > b;
+  if (ok) {
     m.th = tn;
     m.re_old = re_new;
-    ++m.acc;
   }
+  return ok;
 }
 
 // The same hit in binary32, from the same two words.  The proposal is
@@ -149,18 +149,19 @@ __device__ __forceinline__ void metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // (bl2e = beta log2(e)): one v_exp_f32 and the word converted without scaling.
 // thm is 1 for theta + delta and 0 for the independent proposal, so both
 // forms are one more FMA (theta * thm + offset, rounded once).
-__device__ __forceinline__ void metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, float thm,
+__device__ __forceinline__ bool metro_hit_f32(MetroState<float>& m, Staple<float> S, float bl2e, float thm,
                                               float pscale, float poff, uint32_t wu, uint32_t wb) {
   const float tn = __builtin_fmaf(m.th, thm, __builtin_fmaf((float)wu, pscale, -poff));
   float sn, cn;
   UMath<float>::sincos(tn, &sn, &cn);
   const float re_new = __builtin_fmaf(S.re, cn, -(S.im * sn));
   const float x = __builtin_fmaf(bl2e, re_new - m.re_old, 32.0f);
-  if (__builtin_amdgcn_exp2f(x) > (float)wb) {
+  const bool ok = __builtin_amdgcn_exp2f(x) > (float)wb;
+  if (ok) {
     m.th = tn;
     m.re_old = re_new;
-    ++m.acc;
   }
+  return ok;
 }
 
 // SPT even/odd sites per thread: sites chunk*SPT*bdim + k*bdim + tid, k < SPT,
@@ -202,10 +203,13 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
                                      (Real)r.inv_aniso_f, aniso, &pp[k]);
     th[k] = ld(pmu, same[k]);
   }
-  int acc = 0;
+  // accepted hits of this wavefront (valid sites only): a wave-uniform count
+  // from the accept masks, kept on the scalar unit
+  uint32_t acc_wave = 0;
   if (KIND == KIND_METROPOLIS) {
     MetroState<Real> m[SPT];
     uint32_t l[SPT];
+    uint64_t vmask[SPT];  // lanes with a valid site
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
       m[k].th = th[k];
@@ -215,8 +219,8 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
         m[k].re_old = __builtin_fmaf(S[k].re, c0, -(S[k].im * s0));
       else
         m[k].re_old = S[k].re * c0 - S[k].im * s0;
-      m[k].acc = 0;
       l[k] = same[k] + (uint32_t)MU * (uint32_t)r.V;
+      vmask[k] = wave_mask(ps[k].valid);
     }
     const uint32_t sweep = a.sweep_dev ? *a.sweep_dev * a.sweep_mul + a.sweep : a.sweep;
     const Real beta = (Real)r.beta;
@@ -231,20 +235,25 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
       for (int k = 0; k < SPT; ++k) {
         const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
         if constexpr (sizeof(Real) == sizeof(float)) {
-          metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[0], w.v[1]);
-          if (h + 1 < a.nhit) metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[2], w.v[3]);
+          const bool ok0 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[0], w.v[1]);
+          acc_wave += count_wave(ok0, vmask[k]);
+          if (h + 1 < a.nhit) {
+            const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[2], w.v[3]);
+            acc_wave += count_wave(ok1, vmask[k]);
+          }
           m[k].th = u1m::wrap_f(m[k].th);  // back into [-pi, pi] once per pair
         } else {
-          metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
-          if (h + 1 < a.nhit) metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+          const bool ok0 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
+          acc_wave += count_wave(ok0, vmask[k]);
+          if (h + 1 < a.nhit) {
+            const bool ok1 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+            acc_wave += count_wave(ok1, vmask[k]);
+          }
         }
       }
     }
 #pragma unroll
-    for (int k = 0; k < SPT; ++k) {
-      th[k] = m[k].th;
-      if (ps[k].valid) acc += m[k].acc;
-    }
+    for (int k = 0; k < SPT; ++k) th[k] = m[k].th;
   } else {
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
@@ -284,10 +293,8 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
       row[3 - c] = 0.0;
     }
   }
-  if (KIND == KIND_METROPOLIS && a.accepts) {
-    const uint64_t tot = wave_sum_u64((uint64_t)acc);
-    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(a.accepts, (unsigned long long)tot);
-  }
+  if (KIND == KIND_METROPOLIS && a.accepts && (threadIdx.x & 63) == 0 && acc_wave)
+    atomicAdd(a.accepts, (unsigned long long)acc_wave);
 }
 
 template <typename Real>
