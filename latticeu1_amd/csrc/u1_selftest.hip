@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void k_math(int fn, const double* __restrict__
       out[2 * i + 1] = c;
       break;
     }
-    default: out[i] = UMath<float>::atan2((float)x[i], (float)y[i]); break;
+    case 8: out[i] = UMath<float>::atan2((float)x[i], (float)y[i]); break;
+    default: out[i] = u1m::wrap_f((float)x[i]); break;
   }
 }
 

@@ -76,8 +76,19 @@ def test_fp32_update_math_accuracy(u1):
     ec = np.max(np.abs(f[:, 1] - np.cos(x)))
     y = rng.uniform(-6, 6, 50000).astype(np.float32).astype(np.float64)
     ea = np.max(np.abs(u1.selftest_math(8, x, y) - np.arctan2(x, y)))
-    say(f"fp32 update math: max |err| sin {es:.2e} cos {ec:.2e} atan2 {ea:.2e}")
+    # binary32 wrap: the angle of the binary64 wrap_d within one binary32 ulp
+    # of pi, in [-pi, pi] up to a few ulps (inputs up to 3 pi + 1, as the OR angle)
+    edges = np.add.outer(np.pi * np.array([-3.0, -1.0, 1.0, 3.0]), 1e-7 * np.arange(-40, 41)).ravel()
+    xw = np.concatenate([rng.uniform(-3 * np.pi - 1, 3 * np.pi + 1, 50000), edges]).astype(np.float32)
+    xw = xw.astype(np.float64)
+    wf = u1.selftest_math(9, xw)
+    ulp_pi = float(np.spacing(np.float32(np.pi)))
+    dw = wf - np.arctan2(np.sin(xw), np.cos(xw))
+    dw -= 2 * np.pi * np.rint(dw / (2 * np.pi))
+    ew = np.max(np.abs(dw))
+    say(f"fp32 update math: max |err| sin {es:.2e} cos {ec:.2e} atan2 {ea:.2e} wrap {ew:.2e}")
     assert es < 5e-6 and ec < 5e-6 and ea < 2e-6
+    assert ew <= ulp_pi and np.max(np.abs(wf)) <= np.pi + 4 * ulp_pi
 
 
 def test_device_philox_kat(u1):
@@ -509,15 +520,27 @@ def test_ensemble_sweep_plaquette(u1, prec):
     b.destroy()
 
 
-def test_ensemble_fp32_runs(u1):
-    geoms = [u1.geom((8, 8, 8, 8), 0.95), u1.geom((12, 12, 12, 16), 1.05)]
-    e = u1.Ensemble(geoms, [1, 2], [1, 0], u1.F32, 1, 3, u1.BENCH_METRO)
+def test_ensemble_fp32_matches_single_lattices(u1):
+    """FP32 has no oracle, so the batched path is pinned to the single-lattice
+    path: replica i of an FP32 ensemble (graph sweeps) equals, bit for bit, an
+    FP32 lattice hot-started and updated alone with the same seed and replica
+    index (Philox counter word 3), anisotropy included."""
+    specs = [((8, 8, 8, 8), 0.95, 1.0, 1), ((12, 12, 12, 16), 1.05, 1.3, 1), ((6, 4, 4, 8), 1.0, 1.0, 0)]
+    geoms = [u1.geom(d, b, x) for d, b, x, _ in specs]
+    seeds = [11, 22, 33]
+    e = u1.Ensemble(geoms, seeds, [h for *_, h in specs], u1.F32, 1, 3, u1.BENCH_METRO)
     e.sweep(20, graph=True)
+    for i, (g, seed, spec) in enumerate(zip(geoms, seeds, specs)):
+        lat = u1.Lattice.cold(g, u1.F32)
+        if spec[3]:
+            lat.hotstart(seed, replica=i)
+        for s in range(20):
+            lat.update(1, 3, u1.BENCH_METRO, seed, s, c3=i)
+        assert np.array_equal(e.read(i), lat.to_host().astype(np.float64)), i
     p = e.plaquette()
-    assert np.all(np.isfinite(p))
     assert 0.2 < u1.mean_plaquette(p[0]) < 0.8 and 0.2 < u1.mean_plaquette(p[1]) < 0.8
     s, n = e.info()
-    assert s == 20 and n == 4 * (8 ** 4 + 12 ** 3 * 16)
+    assert s == 20 and n == 4 * (8 ** 4 + 12 ** 3 * 16 + 6 * 4 * 4 * 8)
     e.destroy()
 
 
