@@ -127,8 +127,6 @@ def constants():
     f = {}
     f["PI"] = to_f32(pi)
     f["TWO_PI"] = to_f32(2 * pi)
-    f["TWO_PI_LO"] = to_f32(2 * pi - Decimal(f["TWO_PI"]))
-    f["INV_TWO_PI"] = to_f32(1 / (2 * pi))
     f["PIO2_HI"], f["PIO2_LO"] = split2(pi / 2, to_f32)
     f["PIO4"] = to_f32(pi / 4)
     f["TWO_OVER_PI"] = to_f32(2 / pi)

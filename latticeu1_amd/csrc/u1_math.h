@@ -143,13 +143,13 @@ U1_HD double wrap_d(double x) {
   return x;
 }
 
-// Binary32 wrap for the binary32 update kernels: x - k 2pi with
-// k = rint(x / 2pi), 2pi split into two binary32 parts (Cody-Waite), so the
-// reduction itself adds at most about one ulp.  The result lies in [-pi, pi]
-// up to binary32 rounding (checked on the CPU against the binary64 wrap_d).
-U1_HD float wrap_f(float x) {
-  const float k = __builtin_rintf(x * KF_INV_TWO_PI);
-  return __builtin_fmaf(-k, KF_TWO_PI_LO, __builtin_fmaf(-k, KF_TWO_PI, x));
+// Wrap of the binary32 update kernels, done in binary64: x - 2pi rint(x / 2pi)
+// (one rint, no correction steps).  The result lies in [-pi, pi] up to
+// binary64 rounding and is the angle of wrap_d(x) within a few binary64 ulps
+// (checked on the CPU); the kernels then round it once to binary32.
+U1_HD double wrap_near_d(double x) {
+  const double k = rintd(x * (K_TWO_OVER_PI * 0.25));
+  return fmad(-k, K_TWO_PI, x);
 }
 
 // ------------------------------------------------------- Bessel I0, I1 (FP64)

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void k_math(int fn, const double* __restrict__
       break;
     }
     case 8: out[i] = UMath<float>::atan2((float)x[i], (float)y[i]); break;
-    default: out[i] = u1m::wrap_f((float)x[i]); break;
+    default: out[i] = (float)u1m::wrap_near_d((double)(float)x[i]); break;
   }
 }
 

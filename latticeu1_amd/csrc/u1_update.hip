@@ -19,10 +19,11 @@
 // with the same u1_math.h functions.  The library is compiled with
 // -ffp-contract=off so that no FMA is formed implicitly.
 //
-// FP32-bulk variant: angles, staples, proposals, the Metropolis weight, the OR
-// angle and the wraps in binary32 with the hardware transcendentals
-// (v_sin_f32, v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs.  It reads
-// the same Philox words as the FP64 path.  Its parity bar is statistical.
+// FP32-bulk variant: angles, staples, proposals, the Metropolis weight and the
+// OR angle in binary32 with the hardware transcendentals (v_sin_f32,
+// v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs; the periodic wrap into
+// [-pi, pi] in binary64 (wrap_near_d), rounded once to binary32.  It reads the
+// same Philox words as the FP64 path.  Its parity bar is statistical.
 #include "u1_kernels.h"
 
 namespace u1d {
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
             const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[2], w.v[3]);
             acc_wave += count_wave(ok1, vmask[k]);
           }
-          m[k].th = u1m::wrap_f(m[k].th);  // back into [-pi, pi] once per pair
+          m[k].th = (float)u1m::wrap_near_d((double)m[k].th);  // back into [-pi, pi] once per pair
         } else {
           const bool ok0 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
           acc_wave += count_wave(ok0, vmask[k]);
@@ -259,8 +260,8 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     for (int k = 0; k < SPT; ++k) {
       // lattice_functions.h:83-89: theta' = -theta - 2 arg(S), wrapped
       const Real t2 = UMath<Real>::atan2(S[k].im, S[k].re);
-      if constexpr (sizeof(Real) == sizeof(float))
-        th[k] = u1m::wrap_f(__builtin_fmaf(-2.0f, t2, -th[k]));
+      if constexpr (sizeof(Real) == sizeof(float))  // -theta - 2 t2 and the wrap in binary64
+        th[k] = (float)u1m::wrap_near_d(__builtin_fma(-2.0, (double)t2, -(double)th[k]));
       else
         th[k] = u1m::wrap_d(-th[k] - 2.0 * t2);
     }

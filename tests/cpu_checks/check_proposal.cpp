@@ -41,35 +41,37 @@ int main() {
   if (bad) return 1;
   std::printf("proposal ok\n");
 
-  // wrap_f (binary32 update kernels) against the binary64 wrap_d of the same
-  // binary32 input on [-3 pi - 1, 3 pi + 1] (the overrelaxation angle reaches
-  // 3 pi, theta + delta 2 pi): the result lies in [-pi, pi] up to a few
-  // binary32 ulps and is the same angle as wrap_d's within one ulp of pi.
+  // wrap_near_d (the binary32 update kernels' wrap, in binary64) against the
+  // stepwise wrap_d on [-3 pi - 1, 3 pi + 1] (the overrelaxation angle
+  // reaches 3 pi, theta + delta 3 pi between two wraps): the same angle within
+  // 4 binary64 ulps of pi, in [-pi, pi] up to that; and, rounded to binary32
+  // as the kernels store it, within one binary32 ulp of pi.
   long wbad = 0, wn = 0;
-  const double ulp_pi = (double)std::nextafter((float)u1m::K_PI, INFINITY) - (double)(float)u1m::K_PI;
-  auto wcheck = [&](float x) {
-    const double a = u1m::wrap_d((double)x);
-    const double b = (double)u1m::wrap_f(x);
+  const double ulp_pi_d = std::nextafter(u1m::K_PI, INFINITY) - u1m::K_PI;
+  const double ulp_pi_f = (double)std::nextafter((float)u1m::K_PI, INFINITY) - (double)(float)u1m::K_PI;
+  auto ang = [](double d) { return d - u1m::K_TWO_PI * std::rint(d / u1m::K_TWO_PI); };
+  auto wcheck = [&](double x) {
+    const double a = u1m::wrap_d(x), b = u1m::wrap_near_d(x);
+    const double bf = (double)(float)u1m::wrap_near_d((double)(float)x), af = u1m::wrap_d((double)(float)x);
     ++wn;
-    double d = b - a;
-    d -= u1m::K_TWO_PI * std::rint(d / u1m::K_TWO_PI);
-    if (std::fabs(b) > u1m::K_PI + 4 * ulp_pi || std::fabs(d) > ulp_pi) {
-      if (wbad < 5) std::printf("wrap mismatch x=%.9g: %.17g vs %.17g\n", (double)x, a, b);
+    if (std::fabs(b) > u1m::K_PI + 4 * ulp_pi_d || std::fabs(ang(b - a)) > 4 * ulp_pi_d ||
+        std::fabs(ang(bf - af)) > ulp_pi_f) {
+      if (wbad < 5) std::printf("wrap mismatch x=%.17g: %.17g vs %.17g (binary32 %.9g)\n", x, a, b, bf);
       ++wbad;
     }
   };
-  const float lim = (float)(3.0 * u1m::K_PI + 1.0);
-  std::uniform_real_distribution<float> ux(-lim, lim);
+  const double lim = 3.0 * u1m::K_PI + 1.0;
+  std::uniform_real_distribution<double> ux(-lim, lim);
   for (int i = 0; i < (1 << 22); ++i) wcheck(ux(rng));
-  for (int m = -3; m <= 3; m += 2)  // odd multiples of pi: 256 binary32 neighbours each side
-    for (float dir : {-INFINITY, INFINITY}) {
-      float y = (float)(m * u1m::K_PI);
+  for (int m = -3; m <= 3; m += 2)  // odd multiples of pi: 256 binary64 neighbours each side
+    for (double dir : {-INFINITY, INFINITY}) {
+      double y = m * u1m::K_PI;
       for (int s = 0; s < 256; ++s) {
         wcheck(y);
         y = std::nextafter(y, dir);
       }
     }
-  std::printf("wrap: %ld of %ld out of range or beyond one ulp of pi\n", wbad, wn);
+  std::printf("wrap: %ld of %ld out of range or off the wrap_d angle\n", wbad, wn);
   if (wbad) return 1;
   std::printf("wrap ok\n");
   return 0;

@@ -28,9 +28,9 @@ def test_device_indexing_on_cpu(tmp_path):
 
 def test_fma_proposal_matches_two_step_form(tmp_path):
     """The kernels' one-FMA Metropolis proposal offset is bit-identical to the
-    oracle's two-step a (2u - 1), and the binary32 kernels' wrap_f lands in
-    [-pi, pi] (up to a few binary32 ulps) on the angle wrap_d gives, within one
-    binary32 ulp of pi (tests/cpu_checks/check_proposal.cpp)."""
+    oracle's two-step a (2u - 1), and the binary32 kernels' binary64 wrap
+    wrap_near_d lands in [-pi, pi] on the angle wrap_d gives, within a few
+    binary64 ulps (tests/cpu_checks/check_proposal.cpp)."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
