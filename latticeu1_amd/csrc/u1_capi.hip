@@ -162,6 +162,12 @@ RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
 }
 
 void set_blocks(RepDesc& r, int block) { u1d::set_phase_blocks(r, block); }
+// Single-lattice update launches take their planes in 2 x 4 tiles (one per XCD).
+constexpr bool kPlaneTiles = true;
+void set_update_blocks(RepDesc& r, int block) {
+  u1d::set_phase_blocks(r, block);
+  u1d::set_phase_tiles(r, kPlaneTiles);
+}
 
 size_t elem_size(int prec) { return prec == U1_F64 ? sizeof(double) : sizeof(float); }
 
@@ -290,7 +296,7 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
       a.accepts = nullptr;
       a.meas = nullptr;
       a.xcd = (uint32_t)c.xcd;
-      set_blocks(a.single, c.block * c.spt);
+      set_update_blocks(a.single, c.block * c.spt);
       float ms = 0;
       RET(time_phase(prec, kind, scratch.p, a, a.single.nblk, c, &ms));
       tbest[i] = std::min(tbest[i], ms);
@@ -348,8 +354,8 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
   if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &co));
   RepDesc rm = a.single, ro = a.single;
-  set_blocks(rm, cm.block * cm.spt);
-  set_blocks(ro, co.block * co.spt);
+  set_update_blocks(rm, cm.block * cm.spt);
+  set_update_blocks(ro, co.block * co.spt);
   if (meas_dst && (ml != 0 || metrop + ovrn == 0)) return fail(U1_ERR_ARG, "no measuring pass");
   const int meas_nblk = ovrn > 0 ? ro.nblk : rm.nblk;  // rows per measuring phase
   if (meas_dst) RET(ensure_fold(3 * meas_nblk, 4));
@@ -760,7 +766,7 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
   a.parity = parity;
   Cfg c = kDefaultCfg;
   RET(phase_cfg(prec, kind, g, lat, a, &c));
-  set_blocks(a.single, c.block * c.spt);
+  set_update_blocks(a.single, c.block * c.spt);
   a.xcd = (uint32_t)c.xcd;
   HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, c.block, c.spt, G.st));
   return U1_OK;

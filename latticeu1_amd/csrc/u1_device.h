@@ -40,6 +40,13 @@ struct RepDesc {
   double aniso;
   uint64_t seed;
   uint64_t lat_off;  // element offset of this replica's angles in the packed buffer
+  // Update launches of a single lattice: 1 if the (z, t) planes are taken in
+  // 2 x 4 tiles of tile_bz x tile_bt planes (set_phase_tiles), one tile per XCD
+  uint32_t tile;
+  uint32_t tile_bz, tile_bt;          // Nz / 2, Nt / 4
+  uint32_t div_tile_m, div_tile_s;    // magic division by tile_bz * tile_bt
+  uint32_t div_bz_m, div_bz_s;        // magic division by tile_bz
+  uint32_t pad1;
 };
 
 // Round-up magic division (Granlund & Montgomery 1994): for 1 <= d < 2^31 and
@@ -102,10 +109,11 @@ __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
 }
 
 // Update-launch decomposition: workgroup b (relative to the replica's first)
-// covers chunk (b % bpp) of the (z, t) plane b / bpp; thread tid of a
-// workgroup of bdim threads takes even/odd site id = plane * P + chunk * bdim
-// + tid, P = Nx/2 * Ny.  z, t (and so the z/t neighbour offsets) are uniform
-// across the workgroup.  valid is false for the tail threads of a plane,
+// covers chunk (b % bpp) of plane b / bpp in launch order, which is the
+// lexicographic plane z + Nz t, or with tiling (set_phase_tiles) the planes of
+// one 2 x 4 tile after another; thread tid of a workgroup of bdim threads takes
+// even/odd site id = (z + Nz t) * P + chunk * bdim + tid, P = Nx/2 * Ny.  z, t
+// (and so the z/t neighbour offsets) are uniform across the workgroup.  valid is false for the tail threads of a plane,
 // which are given the plane's last site.
 struct PhaseSite {
   bool valid;
@@ -116,10 +124,23 @@ struct PhaseSite {
 
 __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b, uint32_t tid, uint32_t bdim,
                                                     int32_t parity, PhaseSite& s) {
-  const uint32_t plane = mdiv(b, r.div_bpp_m, r.div_bpp_s);
+  const uint32_t plane = mdiv(b, r.div_bpp_m, r.div_bpp_s);  // in launch order
   const uint32_t chunk = b - plane * (uint32_t)r.bpp;
-  const uint32_t t = mdiv(plane, r.div_nz_m, r.div_nz_s);
-  const int32_t z = (int32_t)(plane - t * (uint32_t)r.N[2]);
+  uint32_t t;
+  int32_t z;
+  if (r.tile) {
+    // tile k = 2 tb + zb covers z in [zb Bz, zb Bz + Bz), t in [tb Bt, tb Bt + Bt),
+    // its planes in t-major order
+    const uint32_t area = r.tile_bz * r.tile_bt;
+    const uint32_t k = mdiv(plane, r.div_tile_m, r.div_tile_s);
+    const uint32_t w = plane - k * area;
+    const uint32_t wt = mdiv(w, r.div_bz_m, r.div_bz_s);
+    t = (k >> 1) * r.tile_bt + wt;
+    z = (int32_t)((k & 1u) * r.tile_bz + (w - wt * r.tile_bz));
+  } else {
+    t = mdiv(plane, r.div_nz_m, r.div_nz_s);
+    z = (int32_t)(plane - t * (uint32_t)r.N[2]);
+  }
   const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
   const uint32_t i_raw = chunk * bdim + tid;
   s.valid = i_raw < P;
@@ -130,7 +151,7 @@ __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b
   s.z = z;
   const uint32_t y = mdiv(i, r.div_nxh_m, r.div_nxh_s);
   const int32_t xh = (int32_t)(i - y * (uint32_t)r.nxh);
-  s.id = plane * P + i;
+  s.id = ((uint32_t)z + (uint32_t)r.N[2] * t) * P + i;  // lexicographic plane z + Nz t
   const int32_t xodd = ((int32_t)y + z + (int32_t)t + parity) & 1;
   s.fwd[0] = xodd ? ((xh == r.nxh - 1) ? -(r.nxh - 1) : 1) : 0;
   s.bwd[0] = xodd ? 0 : ((xh == 0) ? (r.nxh - 1) : -1);
@@ -161,6 +182,23 @@ inline void set_phase_blocks(RepDesc& r, int block) {
   r.div_bpp_m = m.m;
   r.div_bpp_s = m.s;
   r.nblk = r.bpp * r.N[2] * r.N[3];
+}
+
+// Host helper: take the (z, t) planes of a single lattice in 2 x 4 tiles
+// (Nz / 2 x Nt / 4 planes), in the order of the tiles' XCDs under xcd_remap:
+// each XCD then holds a tile, whose halo to the other XCDs is smaller than
+// that of an Nt / 8-slice slab.  Needs Nt % 4 == 0 (Nz is even); otherwise,
+// and for batched launches, planes stay in lexicographic order.
+inline void set_phase_tiles(RepDesc& r, bool on) {
+  r.tile = on && r.N[2] >= 2 && r.N[3] >= 4 && r.N[3] % 4 == 0 ? 1u : 0u;
+  if (!r.tile) return;
+  r.tile_bz = (uint32_t)r.N[2] / 2;
+  r.tile_bt = (uint32_t)r.N[3] / 4;
+  const MagicDiv a = make_magic(r.tile_bz * r.tile_bt), b = make_magic(r.tile_bz);
+  r.div_tile_m = a.m;
+  r.div_tile_s = a.s;
+  r.div_bz_m = b.m;
+  r.div_bz_s = b.s;
 }
 
 // ------------------------------------------------------- precision traits

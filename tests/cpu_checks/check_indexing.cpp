@@ -88,48 +88,52 @@ int main() {
     r.sy = r.nxh; r.sz = r.nxh * G[1]; r.st = r.nxh * G[1] * G[2];
     u1d::MagicDiv a = u1d::make_magic(r.nxh), b = u1d::make_magic(G[1]), c = u1d::make_magic(G[2]);
     r.div_nxh_m = a.m; r.div_nxh_s = a.s; r.div_ny_m = b.m; r.div_ny_s = b.s; r.div_nz_m = c.m; r.div_nz_s = c.s;
-    for (int B : blocks) {
-      if (r.V > 1000000 && B != 192 && B != 256 && B != 384) continue;
-      u1d::set_phase_blocks(r, B);
-      for (int parity = 0; parity < 2; ++parity) {
-        std::vector<unsigned char> seen(r.HV, 0);
-        for (int blk = 0; blk < r.nblk; ++blk)
-          for (int tid = 0; tid < B; ++tid) {
-            u1d::PhaseSite ps;
-            u1d::phase_site(r, blk, tid, B, parity, ps);
-            if (!ps.valid) {
-              // tail threads load (and never store): every address in bounds
-              for (int d = 0; d < 4; ++d) {
-                const long f = (long)ps.id + ps.fwd[d], bk = (long)ps.id + ps.bwd[d];
-                const long ff = f + ps.fwd[(d + 1) % 4], fb = f + ps.bwd[(d + 1) % 4];
-                if (ps.id >= (uint32_t)r.HV || f < 0 || f >= r.HV || bk < 0 || bk >= r.HV || ff < 0 ||
-                    ff >= r.HV || fb < 0 || fb >= r.HV) {
-                  printf("phase_site: tail thread out of bounds\n");
-                  return 1;
+    for (int tiled = 0; tiled < 2; ++tiled) {
+      for (int B : blocks) {
+        if (r.V > 1000000 && B != 192 && B != 256 && B != 384) continue;
+        u1d::set_phase_blocks(r, B);
+        u1d::set_phase_tiles(r, tiled != 0);
+        if (tiled && !r.tile) continue;  // Nt % 4 != 0: no tiling
+        for (int parity = 0; parity < 2; ++parity) {
+          std::vector<unsigned char> seen(r.HV, 0);
+          for (int blk = 0; blk < r.nblk; ++blk)
+            for (int tid = 0; tid < B; ++tid) {
+              u1d::PhaseSite ps;
+              u1d::phase_site(r, blk, tid, B, parity, ps);
+              if (!ps.valid) {
+                // tail threads load (and never store): every address in bounds
+                for (int d = 0; d < 4; ++d) {
+                  const long f = (long)ps.id + ps.fwd[d], bk = (long)ps.id + ps.bwd[d];
+                  const long ff = f + ps.fwd[(d + 1) % 4], fb = f + ps.bwd[(d + 1) % 4];
+                  if (ps.id >= (uint32_t)r.HV || f < 0 || f >= r.HV || bk < 0 || bk >= r.HV || ff < 0 ||
+                      ff >= r.HV || fb < 0 || fb >= r.HV) {
+                    printf("phase_site: tail thread out of bounds\n");
+                    return 1;
+                  }
                 }
+                continue;
               }
-              continue;
+              if (ps.id >= (uint32_t)r.HV || seen[ps.id]) { printf("phase_site: bad/duplicate id %u\n", ps.id); return 1; }
+              seen[ps.id] = 1;
+              // brute-force coordinates of (id, parity): indexEO restated
+              int x[4];
+              const int za = ps.id / (G[0] / 2), zb = za / G[1];
+              x[1] = za - zb * G[1]; x[3] = zb / G[2]; x[2] = zb - x[3] * G[2];
+              x[0] = 2 * (int)ps.id + ((x[1] + x[2] + x[3] + parity) & 1) - za * G[0];
+              if ((int)ps.t != x[3] || ps.z != x[2]) { printf("phase_site: z/t wrong\n"); return 1; }
+              for (int d = 0; d < 4; ++d)
+                for (int sgn = -1; sgn <= 1; sgn += 2) {
+                  int y[4] = {x[0], x[1], x[2], x[3]};
+                  y[d] = (y[d] + sgn + G[d]) % G[d];
+                  const int want = lexi(G, y) >> 1;
+                  const int got = (int)ps.id + (sgn > 0 ? ps.fwd[d] : ps.bwd[d]);
+                  if (got != want) { printf("phase_site: neighbour d=%d sgn=%d wrong\n", d, sgn); return 1; }
+                }
+              ++psites;
             }
-            if (ps.id >= (uint32_t)r.HV || seen[ps.id]) { printf("phase_site: bad/duplicate id %u\n", ps.id); return 1; }
-            seen[ps.id] = 1;
-            // brute-force coordinates of (id, parity): indexEO restated
-            int x[4];
-            const int za = ps.id / (G[0] / 2), zb = za / G[1];
-            x[1] = za - zb * G[1]; x[3] = zb / G[2]; x[2] = zb - x[3] * G[2];
-            x[0] = 2 * (int)ps.id + ((x[1] + x[2] + x[3] + parity) & 1) - za * G[0];
-            if ((int)ps.t != x[3] || ps.z != x[2]) { printf("phase_site: z/t wrong\n"); return 1; }
-            for (int d = 0; d < 4; ++d)
-              for (int sgn = -1; sgn <= 1; sgn += 2) {
-                int y[4] = {x[0], x[1], x[2], x[3]};
-                y[d] = (y[d] + sgn + G[d]) % G[d];
-                const int want = lexi(G, y) >> 1;
-                const int got = (int)ps.id + (sgn > 0 ? ps.fwd[d] : ps.bwd[d]);
-                if (got != want) { printf("phase_site: neighbour d=%d sgn=%d wrong\n", d, sgn); return 1; }
-              }
-            ++psites;
-          }
-        for (int k = 0; k < r.HV; ++k)
-          if (!seen[k]) { printf("phase_site: site %d not covered (B=%d)\n", k, B); return 1; }
+          for (int k = 0; k < r.HV; ++k)
+            if (!seen[k]) { printf("phase_site: site %d not covered (B=%d)\n", k, B); return 1; }
+        }
       }
     }
   }
