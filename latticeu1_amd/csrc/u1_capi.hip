@@ -25,7 +25,7 @@ thread_local std::string g_err;
 // Launch shape of an update phase kernel: workgroup size, even/odd sites per
 // thread, XCD-aware workgroup order.
 struct Cfg {
-  int block, spt, xcd;
+  int block, spt, xcd, tile = 0;  // tile: single-lattice planes in 2 x 4 tiles (set_phase_tiles)
 };
 
 struct Ctx {
@@ -162,11 +162,11 @@ RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
 }
 
 void set_blocks(RepDesc& r, int block) { u1d::set_phase_blocks(r, block); }
-// Single-lattice update launches take their planes in 2 x 4 tiles (one per XCD).
-constexpr bool kPlaneTiles = true;
-void set_update_blocks(RepDesc& r, int block) {
-  u1d::set_phase_blocks(r, block);
-  u1d::set_phase_tiles(r, kPlaneTiles);
+// Single-lattice update launches: plane decomposition of a launch shape,
+// planes in 2 x 4 tiles (one per XCD) if the shape asks for it.
+void set_update_blocks(RepDesc& r, const Cfg& c) {
+  u1d::set_phase_blocks(r, c.block * c.spt);
+  u1d::set_phase_tiles(r, c.tile != 0);
 }
 
 size_t elem_size(int prec) { return prec == U1_F64 ? sizeof(double) : sizeof(float); }
@@ -230,17 +230,22 @@ void plaquette_shape(const RepDesc& r, int prec, int* block, int* spt) {
 const Cfg kShapes[] = {{64, 1, 0},  {128, 1, 0}, {192, 1, 0}, {256, 1, 0}, {384, 1, 0},
                        {512, 1, 0}, {64, 2, 0},  {128, 2, 0}, {192, 2, 0}, {256, 2, 0},
                        {384, 2, 0}, {64, 4, 0},  {128, 4, 0}, {192, 4, 0}, {256, 4, 0}};
-// every shape is tried with and without the XCD-aware workgroup order
-std::vector<Cfg> candidates() {
+// every shape is tried with and without the XCD-aware workgroup order and,
+// for a single lattice, with lexicographic or 2 x 4 tiled planes: the tiles
+// cut the halo between XCDs while the lattice lives in the MALL, and have
+// measured slower on a lattice streamed from HBM (96^4)
+std::vector<Cfg> candidates(bool tiles) {
   std::vector<Cfg> v;
-  for (int x = 0; x < 2; ++x)
-    for (Cfg c : kShapes) {
-      c.xcd = x;
-      v.push_back(c);
-    }
+  for (int t = 0; t < (tiles ? 2 : 1); ++t)
+    for (int x = 0; x < 2; ++x)
+      for (Cfg c : kShapes) {
+        c.xcd = x;
+        c.tile = t;
+        v.push_back(c);
+      }
   return v;
 }
-const Cfg kDefaultCfg = {256, 1, 1};
+const Cfg kDefaultCfg = {256, 1, 1, 0};
 
 std::string vol_str(const u1_geom* g) {
   std::ostringstream s;
@@ -286,7 +291,7 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
   // Three rounds over all candidates, best time per candidate: a round-robin
   // order keeps clock ramp-up and neighbour noise from favouring one shape.
   std::vector<Cfg> cands;
-  for (const Cfg c : candidates())
+  for (const Cfg c : candidates(true))
     if (!(c.spt == 4 && prec != U1_F32)) cands.push_back(c);
   std::vector<float> tbest(cands.size(), 1e30f);
   for (int round = 0; round < 3; ++round)
@@ -296,7 +301,7 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
       a.accepts = nullptr;
       a.meas = nullptr;
       a.xcd = (uint32_t)c.xcd;
-      set_update_blocks(a.single, c.block * c.spt);
+      set_update_blocks(a.single, c);
       float ms = 0;
       RET(time_phase(prec, kind, scratch.p, a, a.single.nblk, c, &ms));
       tbest[i] = std::min(tbest[i], ms);
@@ -306,12 +311,13 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
     if (tbest[i] < tbest[ib]) ib = i;
   if (G.verbosity >= 2)
     for (size_t i = 0; i < cands.size(); ++i)
-      std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order: %.2f us per launch%s\n", key.c_str(),
-                   cands[i].block, cands[i].spt, cands[i].xcd ? "XCD" : "plain", 1e3 * tbest[i] / 8.0,
-                   i == ib ? "  <- chosen" : "");
+      std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order%s: %.2f us per launch%s\n", key.c_str(),
+                   cands[i].block, cands[i].spt, cands[i].xcd ? "XCD" : "plain", cands[i].tile ? ", tiled planes" : "",
+                   1e3 * tbest[i] / 8.0, i == ib ? "  <- chosen" : "");
   else if (G.verbosity >= 1)
-    std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order: %.2f us per launch\n", key.c_str(),
-                 cands[ib].block, cands[ib].spt, cands[ib].xcd ? "XCD" : "plain", 1e3 * tbest[ib] / 8.0);
+    std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order%s: %.2f us per launch\n", key.c_str(),
+                 cands[ib].block, cands[ib].spt, cands[ib].xcd ? "XCD" : "plain", cands[ib].tile ? ", tiled planes" : "",
+                 1e3 * tbest[ib] / 8.0);
   G.tunecache[key] = cands[ib];
   *cfg = cands[ib];
   return U1_OK;
@@ -354,8 +360,8 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
   if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &co));
   RepDesc rm = a.single, ro = a.single;
-  set_update_blocks(rm, cm.block * cm.spt);
-  set_update_blocks(ro, co.block * co.spt);
+  set_update_blocks(rm, cm);
+  set_update_blocks(ro, co);
   if (meas_dst && (ml != 0 || metrop + ovrn == 0)) return fail(U1_ERR_ARG, "no measuring pass");
   const int meas_nblk = ovrn > 0 ? ro.nblk : rm.nblk;  // rows per measuring phase
   if (meas_dst) RET(ensure_fold(3 * meas_nblk, 4));
@@ -464,7 +470,7 @@ struct u1_ensemble {
   uint32_t* base_dev = nullptr;  // sweep index of series row 0
   hipGraph_t mgraph = nullptr;
   hipGraphExec_t mexec = nullptr;
-  Cfg cfg = {256, 1, 1};  // launch shape of the update phases (tuned on first use)
+  Cfg cfg = {256, 1, 1, 0};  // launch shape of the update phases (tuned on first use)
   bool tuned = false;
 };
 
@@ -557,7 +563,7 @@ int ensemble_tune(u1_ensemble* e) {
   HIPCHK(es.create(&e1));
   float best = 1e30f;
   Cfg bestc = e->cfg;
-  for (const Cfg c : candidates()) {
+  for (const Cfg c : candidates(false)) {  // batched launches keep lexicographic planes
     if (c.spt == 4 && e->prec != U1_F32) continue;
     RET(ensemble_build_update(e, c));
     RET(ensemble_enqueue_phases(e, scratch.p));  // warm-up
@@ -766,7 +772,7 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
   a.parity = parity;
   Cfg c = kDefaultCfg;
   RET(phase_cfg(prec, kind, g, lat, a, &c));
-  set_update_blocks(a.single, c.block * c.spt);
+  set_update_blocks(a.single, c);
   a.xcd = (uint32_t)c.xcd;
   HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, c.block, c.spt, G.st));
   return U1_OK;
@@ -1343,9 +1349,9 @@ int u1_tune_load(const char* path) {
     const bool f32 = name.size() > 4 && name.compare(name.size() - 4, 4, "_f32") == 0;
     bool known = false;
     for (const Cfg& c : kShapes) known = known || (c.block == bx && c.spt == by);
-    if (!known || (by == 4 && !f32) || (bz != 1 && bz != 2))
+    if (!known || (by == 4 && !f32) || bz < 1 || bz > 4)
       return fail(U1_ERR_IO, std::string("Bad launch shape in ") + path + ": " + line);
-    parsed.push_back({vol + "\t" + name, Cfg{bx, by, bz == 2 ? 1 : 0}});
+    parsed.push_back({vol + "\t" + name, Cfg{bx, by, (bz - 1) & 1, (bz - 1) >> 1}});
   }
   for (const auto& kv : parsed) G.tunecache[kv.first] = kv.second;
   G.tune_loaded = (int)parsed.size();
@@ -1363,8 +1369,9 @@ int u1_tune_save(const char* path) {
     const size_t tab = key.find('\t');
     const std::string vol = key.substr(0, tab), name = key.substr(tab + 1);
     f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.block << "\t" << kv.second.spt << "\t"
-      << (kv.second.xcd ? 2 : 1)
-      << "\t0\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z = 2: XCD-aware order\n";
+      << 1 + kv.second.xcd + 2 * kv.second.tile
+      << "\t0\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z - 1 = XCD-aware order (1) + "
+         "tiled planes (2)\n";
   }
   return f ? U1_OK : fail(U1_ERR_IO, "write failed");
 }

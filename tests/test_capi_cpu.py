@@ -56,14 +56,18 @@ def test_tune_cache_roundtrip(tmp_path):
     lib = u1.load()
     p = tmp_path / "tunecache.csv"
     p.write_text("tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n"
-                 "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t0\t1\t1\t0\t# x\n")
+                 "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t0\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t0\t1\t1\t0\t# x\n")
     u1.tune_load(str(p))
     n = C.c_int32(0)
-    assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 1
+    assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 2
     q = tmp_path / "out.csv"
     u1.tune_save(str(q))
     txt = q.read_text()
     assert txt.startswith("tunecache\tlatticeu1_amd-1") and "8x8x8x8\tMetropolis_f64" in txt
+    # block.z = 1 + XCD order + 2 x tiled planes survives the round trip
+    assert "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t" in txt
+    assert "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t" in txt
     bad = tmp_path / "bad.csv"
     bad.write_text("not a cache\n")
     with pytest.raises(u1.U1Error):
@@ -74,11 +78,13 @@ def test_tune_cache_roundtrip(tmp_path):
         u1.tune_load(str(stale))
 
 
-@pytest.mark.parametrize("shape", ["100\t1\t1", "0\t1\t1", "1024\t1\t1", "128\t3\t1", "128\t1\t3", "64\t4\t2"])
+@pytest.mark.parametrize("shape", ["100\t1\t1", "0\t1\t1", "1024\t1\t1", "128\t3\t1", "128\t1\t5", "128\t1\t0",
+                                   "64\t4\t2"])
 def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, shape):
     """A cache entry must be one of the tuner's own shapes (whole wavefronts,
-    <= 512 threads, 1/2/4 sites per thread, 4 only for binary32), and a file
-    with one bad line changes nothing."""
+    <= 512 threads, 1/2/4 sites per thread, 4 only for binary32; block.z 1-4 =
+    1 + XCD order + 2 x tiled planes), and a file with one bad line changes
+    nothing."""
     import latticeu1_amd as u1
 
     lib = u1.load()
