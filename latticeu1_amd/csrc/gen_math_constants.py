@@ -6,7 +6,9 @@ module at 100 significant digits and rounded once to binary64 / binary32:
 
 * pi from Machin's formula  pi = 16 atan(1/5) - 4 atan(1/239);
 * ln 2 from the series      ln 2 = 2 atanh(1/3);
-* Taylor coefficients 1/n! and 1/(2k+1) exactly as rationals.
+* Taylor coefficients 1/n! and 1/(2k+1) exactly as rationals;
+* one minimax table, binary32 atan on [0, 1] for the FP32 update kernels,
+  from a Remez exchange (remez_atan01), kept as literals.
 
 Cody-Waite splits (pi/2 = P1 + P2 + P3, ln2 = HI + LO) take the remainder of
 the exact value after subtracting the exactly-represented leading parts, so the
@@ -14,7 +16,8 @@ sum P1 + P2 + P3 carries about 150 bits of pi/2.
 
 Run:  python3 gen_math_constants.py > u1_math_constants.h
 The test suite re-derives every value with these same functions and compares
-against the committed header, so the header cannot drift from this recipe.
+against the committed header, so the header cannot drift from this recipe
+(the minimax table: its error is re-measured and the exchange re-run).
 """
 from decimal import Decimal, getcontext
 from fractions import Fraction
@@ -143,7 +146,7 @@ def inv_fact(n: int) -> Fraction:
 
 
 def coeff_tables():
-    """Taylor coefficient tables, highest order first (Horner order)."""
+    """Coefficient tables, highest order first (Horner order)."""
     t = {}
     # sin r = r + r^3 * P(r^2), P has terms (-1)^k / (2k+3)!, k = 0..7  (through r^17)
     t["SIN64"] = [to_f64(inv_fact(2 * k + 3) * (-1) ** (k + 1)) for k in range(7, -1, -1)]
@@ -157,7 +160,72 @@ def coeff_tables():
     t["SIN32"] = [to_f32(inv_fact(2 * k + 3) * (-1) ** (k + 1)) for k in range(3, -1, -1)]  # through r^9
     t["COS32"] = [to_f32(inv_fact(2 * k + 2) * (-1) ** (k + 1)) for k in range(4, -1, -1)]  # through r^10
     t["ATAN32"] = [to_f32(Fraction((-1) ** (k + 1), 2 * k + 3)) for k in range(8, -1, -1)]  # through t^19
+    # minimax atan z = z + z^3 A(z^2) on all of [0, 1] (binary32 update kernels)
+    t["ATAN01_MM32"] = [float.fromhex(h) for h in ATAN01_MM32_HEX]
     return t
+
+
+# Minimax coefficients of atan z = z + z^3 A(z^2) on 0 <= z <= 1, relative
+# error weight, A of degree 7 in w = z^2, highest order first, rounded to
+# binary32.  They are the output of remez_atan01(8) below, rounded once; the
+# literal values are kept so the header does not depend on the LAPACK build.
+# Max relative error of the polynomial: 1.67e-8 before rounding, 1.81e-8 after
+# (the test suite re-measures it).
+ATAN01_MM32_HEX = ["0x1.7ed232p-9", "-0x1.0c2c14p-6", "0x1.61fddap-5", "-0x1.3556b6p-4",
+                   "0x1.b4e12ap-4", "-0x1.230adcp-3", "0x1.9978f4p-3", "-0x1.5554dcp-2"]
+
+
+def _atan01_q(w):
+    """(atan z - z) / z^3 at w = z^2, with its Taylor series near 0."""
+    import numpy as np
+    w = np.asarray(w, dtype=np.float64)
+    z = np.sqrt(w)
+    out = np.empty_like(w)
+    small = w < 1e-3
+    ws = w[small]
+    out[small] = -1 / 3 + ws / 5 - ws**2 / 7 + ws**3 / 9 - ws**4 / 11 + ws**5 / 13
+    zb = z[~small]
+    out[~small] = (np.arctan(zb) - zb) / zb**3
+    return out
+
+
+def remez_atan01(n: int, iters: int = 30, grid: int = 200001):
+    """Remez exchange (Remez 1934) for the n coefficients of A in
+    atan z = z + z^3 A(z^2), minimising max |atan z - approx| / atan z over
+    [0, 1].  A is carried in the Chebyshev basis of 2w - 1 for conditioning;
+    the reference set moves to the n + 1 alternating extrema of the weighted
+    error on a grid of w.  Returns (coefficients in w, lowest order first,
+    as binary64; the levelled error)."""
+    import numpy as np
+    from numpy.polynomial import chebyshev as cheb
+    from numpy.polynomial import polynomial as poly
+
+    def weight(w):  # error of A times this = relative error of atan
+        z = np.sqrt(w)
+        return np.where(w > 0, w * z / np.arctan(np.where(w > 0, z, 1.0)), 0.0)
+
+    xs = np.linspace(0.0, 1.0, grid)
+    ref = np.sort(0.5 - 0.5 * np.cos(np.pi * np.arange(n + 1) / n))
+    ref[0] = 0.02  # the weight vanishes at w = 0
+    for _ in range(iters):
+        a = np.zeros((n + 1, n + 1))
+        a[:, :n] = cheb.chebvander(2 * ref - 1, n - 1)
+        a[:, n] = (-1.0) ** np.arange(n + 1) / weight(ref)
+        sol = np.linalg.solve(a, _atan01_q(ref))
+        c, lev = sol[:n], abs(sol[n])
+        err = weight(xs) * (_atan01_q(xs) - cheb.chebval(2 * xs - 1, c))
+        # the largest |error| of each run of equal sign
+        starts = np.concatenate([[0], np.nonzero(np.diff(np.sign(err)))[0] + 1])
+        ends = np.concatenate([starts[1:], [grid]])
+        idx = [a + int(np.argmax(np.abs(err[a:b]))) for a, b in zip(starts, ends)]
+        while len(idx) > n + 1:
+            idx = idx[1:] if abs(err[idx[0]]) < abs(err[idx[-1]]) else idx[:-1]
+        ref = xs[idx]
+    mono = cheb.cheb2poly(c)  # in u = 2w - 1
+    cw = np.zeros(n)
+    for k, m in enumerate(mono):
+        cw[:k + 1] += m * poly.polypow([-1.0, 2.0], k)[:k + 1]
+    return cw, lev
 
 
 def render() -> str:
@@ -166,7 +234,8 @@ def render() -> str:
     out = []
     out.append("// Generated by gen_math_constants.py -- do not edit by hand.")
     out.append("// Each value is the correctly rounded binary64/binary32 image of an exact")
-    out.append("// mathematical constant computed with 100-digit decimal arithmetic.")
+    out.append("// mathematical constant computed with 100-digit decimal arithmetic, except")
+    out.append("// the ATAN01_MM32 table (minimax coefficients from a Remez exchange).")
     out.append("#pragma once")
     out.append("")
     out.append("namespace u1m {")

@@ -156,6 +156,8 @@ RepDesc make_rep(const u1_geom* g, uint64_t seed, uint32_t c3) {
   r.beta = g->beta;
   r.aniso = g->aniso;
   r.inv_aniso_f = (float)(1.0 / g->aniso);
+  r.aniso_f = (float)g->aniso;
+  r.bl2e_f = (float)(g->beta * 1.4426950408889634);  // beta log2(e)
   r.seed = seed;
   r.lat_off = 0;
   return r;

@@ -46,7 +46,9 @@ struct RepDesc {
   uint32_t tile_bz, tile_bt;          // Nz / 2, Nt / 4
   uint32_t div_tile_m, div_tile_s;    // magic division by tile_bz * tile_bt
   uint32_t div_bz_m, div_bz_s;        // magic division by tile_bz
-  uint32_t pad1;
+  // binary32 update constants, rounded once on the host: aniso, and
+  // beta log2(e) for the exp2 form of the Metropolis weight
+  float aniso_f, bl2e_f;
 };
 
 // Round-up magic division (Granlund & Montgomery 1994): for 1 <= d < 2^31 and
@@ -241,21 +243,21 @@ struct UMath<float> {
     *c = __builtin_amdgcn_cosf(rev);
   }
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
-  // atan2 with the octant reduction of u1_math.h, divisions by v_rcp_f32
+  // atan2: octant reduction to z = min/max in [0, 1] (division by v_rcp_f32),
+  // then a minimax polynomial atan z = z + z^3 A(z^2) on all of [0, 1]
+  // (relative error 1.8e-8, gen_math_constants.py: remez_atan01), so no
+  // second reduction by tan(pi/8) is needed
   static __device__ __forceinline__ float atan2(float y, float x) {
     const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
     const bool swap = ay > ax;
     const float num = swap ? ax : ay, den = swap ? ay : ax;
-    float z = den > 0.0f ? num * __builtin_amdgcn_rcpf(den) : 0.0f;
-    const bool red = z > u1m::KF_TAN_PIO8;
-    const float t = red ? (z - 1.0f) * __builtin_amdgcn_rcpf(z + 1.0f) : z;
-    const float t2 = t * t;
-    const float ca[u1m::N_ATAN32] = {U1M_ATAN32_COEFFS};
+    const float z = den > 0.0f ? num * __builtin_amdgcn_rcpf(den) : 0.0f;
+    const float w = z * z;
+    const float ca[u1m::N_ATAN01_MM32] = {U1M_ATAN01_MM32_COEFFS};
     float p = ca[0];
 #pragma unroll
-    for (int i = 1; i < u1m::N_ATAN32; ++i) p = __builtin_fmaf(p, t2, ca[i]);
-    float a = __builtin_fmaf(t * t2, p, t);
-    if (red) a += u1m::KF_PIO4;
+    for (int i = 1; i < u1m::N_ATAN01_MM32; ++i) p = __builtin_fmaf(p, w, ca[i]);
+    float a = __builtin_fmaf(z * w, p, z);
     if (swap) a = (u1m::KF_PIO2_HI - a) + u1m::KF_PIO2_LO;
     if (__builtin_signbit(x)) a = u1m::KF_PI - a;
     return __builtin_signbit(y) ? -a : a;

@@ -19,6 +19,9 @@ struct PhaseArgs {
   int32_t proposal;             // 0 independent, 1 theta + delta
   int32_t nhit;
   double step;
+  // binary32 Metropolis proposal a (2 wu 2^-32 - 1) + thm theta, a = pi or
+  // step: pscale = a 2^-31, poff = a, thm = 0 or 1 (set by launch_phase)
+  float pscale_f, poff_f, thm_f;
   uint32_t stream, pass, sweep;
   uint32_t xcd;                 // 1: XCD-aware workgroup order (xcd_remap)
   const uint32_t* sweep_dev;    // if set, sweep index = *sweep_dev * sweep_mul + sweep (graph replays)

@@ -95,13 +95,15 @@ __device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, 
       ssi = first ? ai : ssi + ai;
     }
   }
-  if (sizeof(Real) == sizeof(float) && MU == 3) {  // no space-space staples
-    if (!aniso) return {str, sti};
-    return {str * xi, sti * xi};
+  if constexpr (sizeof(Real) == sizeof(float)) {
+    // binary32: the weighted form with no test of xi; at xi == 1 both weights
+    // are exactly 1 and the fused sum equals the plain sum ss + st
+    if (MU == 3) return {str * xi, sti * xi};  // no space-space staples
+    return {__builtin_fmaf(ssr, inv_xi, str * xi), __builtin_fmaf(ssi, inv_xi, sti * xi)};
+  } else {
+    if (!aniso) return {ssr + str, ssi + sti};  // xi == 1: identical to ss/1 + st*1
+    return {ssr / xi + str * xi, ssi / xi + sti * xi};
   }
-  if (!aniso) return {ssr + str, ssi + sti};  // xi == 1: identical to ss/1 + st*1
-  if (sizeof(Real) == sizeof(double)) return {ssr / xi + str * xi, ssi / xi + sti * xi};
-  return {ssr * inv_xi + str * xi, ssi * inv_xi + sti * xi};
 }
 
 template <typename Real>
@@ -191,6 +193,7 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   Real* lat = lat_base + r.lat_off;
   Real* pmu = lat + (size_t)MU * r.V;
   const bool aniso = r.aniso != 1.0;
+  const Real xi = sizeof(Real) == sizeof(float) ? (Real)r.aniso_f : (Real)r.aniso;
   uint32_t same[SPT];
   Staple<Real> S[SPT];
   Real th[SPT];
@@ -200,8 +203,8 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     same[k] = (uint32_t)(a.parity * r.HV) + ps[k].id;
     const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
     pp[k] = {0, 0, 0, 0};
-    S[k] = staple_at<Real, MU, MEAS>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, (Real)r.aniso,
-                                     (Real)r.inv_aniso_f, aniso, &pp[k]);
+    S[k] = staple_at<Real, MU, MEAS>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, xi, (Real)r.inv_aniso_f, aniso,
+                                     &pp[k]);
     th[k] = ld(pmu, same[k]);
   }
   // accepted hits of this wavefront (valid sites only): a wave-uniform count
@@ -225,21 +228,19 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
     }
     const uint32_t sweep = a.sweep_dev ? *a.sweep_dev * a.sweep_mul + a.sweep : a.sweep;
     const Real beta = (Real)r.beta;
-    // binary32 hit constants (metro_hit_f32)
-    const float bl2e = (float)(r.beta * 1.4426950408889634);
-    const float amp = a.proposal == 0 ? u1m::KF_PI : (float)a.step;
-    const float pscale = amp * 0x1p-31f;
-    const float thm = a.proposal == 0 ? 0.0f : 1.0f;
+    // binary32 hit constants (metro_hit_f32), rounded on the host
+    const float bl2e = r.bl2e_f;
+    const float pscale = a.pscale_f, poff = a.poff_f, thm = a.thm_f;
     for (int h = 0; h < a.nhit; h += 2) {
       const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
         if constexpr (sizeof(Real) == sizeof(float)) {
-          const bool ok0 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[0], w.v[1]);
+          const bool ok0 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[0], w.v[1]);
           acc_wave += count_wave(ok0, vmask[k]);
           if (h + 1 < a.nhit) {
-            const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, amp, w.v[2], w.v[3]);
+            const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[2], w.v[3]);
             acc_wave += count_wave(ok1, vmask[k]);
           }
           m[k].th = (float)u1m::wrap_near_d((double)m[k].th);  // back into [-pi, pi] once per pair
@@ -354,7 +355,14 @@ hipError_t launch_prec(int kind, int mu, Real* lat, const PhaseArgs& a, int nblo
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
                         int spt, hipStream_t s) {
   if (prec == PREC_F64) return launch_prec<double>(kind, mu, (double*)lat_base, a, nblocks, block, spt, s);
-  return launch_prec<float>(kind, mu, (float*)lat_base, a, nblocks, block, spt, s);
+  // binary32 proposal constants: a (2 wu 2^-32 - 1) with a = pi (independent
+  // angle) or step (theta + delta); a 2^-31 is exact
+  PhaseArgs b = a;
+  const float amp = a.proposal == 0 ? u1m::KF_PI : (float)a.step;
+  b.pscale_f = amp * 0x1p-31f;
+  b.poff_f = amp;
+  b.thm_f = a.proposal == 0 ? 0.0f : 1.0f;
+  return launch_prec<float>(kind, mu, (float*)lat_base, b, nblocks, block, spt, s);
 }
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t s) {

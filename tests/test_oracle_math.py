@@ -140,6 +140,33 @@ def test_constants_header_is_generated():
     assert abs(err) < Decimal(10) ** -45
 
 
+def test_atan_minimax_table():
+    """The binary32 minimax table of the FP32 update kernels' atan2
+    (atan z = z + z^3 A(z^2) on [0, 1]): the rounded polynomial's relative
+    error on a dense grid, the equioscillation level of a fresh Remez run, and
+    the table being that run's result rounded to binary32 (within one unit of
+    the last place, to allow for another LAPACK)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("gen", os.path.join(ROOT, "latticeu1_amd", "csrc",
+                                                                      "gen_math_constants.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    table = np.array([float.fromhex(h) for h in gen.ATAN01_MM32_HEX])  # highest order first
+    z = np.linspace(0.0, 1.0, 1_000_001)[1:]
+    w = z * z
+    rel = np.abs(z + z**3 * np.polyval(table, w) - np.arctan(z)) / np.arctan(z)
+    assert rel.max() < 2e-8
+    # the Taylor series truncated at the same degree is far worse on [0, 1]
+    taylor = np.array([(-1.0) ** (k + 1) / (2 * k + 3) for k in range(7, -1, -1)])
+    assert (np.abs(z + z**3 * np.polyval(taylor, w) - np.arctan(z)) / np.arctan(z)).max() > 1e-2
+    cw, lev = gen.remez_atan01(8)
+    assert 1.6e-8 < lev < 1.75e-8
+    bits_fit = cw[::-1].astype(np.float32).view(np.int32).astype(np.int64)
+    bits_tab = table.astype(np.float32).view(np.int32).astype(np.int64)
+    assert np.max(np.abs(bits_fit - bits_tab)) <= 1
+
+
 def test_libm_variant_agrees_short_time(orc, orc_libm):
     """Same algorithms with glibc math.  One sweep must agree to rounding level
     (identical accept decisions, angles within 1e-7): this pins that the shared
