@@ -401,12 +401,15 @@ def test_fp32_fixed_configuration(u1, orc, thermal):
         assert np.max(np.abs(c32 - c64)) <= 1e-6 * scale, (mode, np.max(np.abs(c32 - c64)) / scale)
 
 
-def test_fp32_overrelaxation_tracks_fp64(u1, orc):
+@pytest.mark.parametrize("aniso", [1.0, 1.7])
+def test_fp32_overrelaxation_tracks_fp64(u1, orc, aniso):
     """From a configuration exactly representable in FP32, one OR sweep in
-    FP32 stays within a few FP32 ulps of the FP64 sweep."""
+    FP32 stays within a few FP32 ulps of the FP64 sweep (the binary32 staple
+    weights space-space terms by 1/xi and space-time terms by xi with no test
+    of xi; at xi = 1 both weights are exactly 1)."""
     dims = (8, 8, 8, 8)
-    g = u1.geom(dims)
-    og = orc.geom(dims)
+    g = u1.geom(dims, 1.0, aniso)
+    og = orc.geom(dims, 1.0, aniso)
     start = orc.hotstart(og, 3).astype(np.float32).astype(np.float64)
     a64 = u1.Lattice.from_host(g, start, u1.F64)
     a32 = u1.Lattice.from_host(g, start, u1.F32)
@@ -416,18 +419,20 @@ def test_fp32_overrelaxation_tracks_fp64(u1, orc):
     assert np.median(np.abs(d)) < 1e-5
 
 
-@pytest.mark.parametrize("proposal,nhit,step,beta", [(0, 1, 0.0, 1.0), (0, 2, 0.0, 0.7), (1, 4, 0.5, 1.0),
-                                                     (1, 3, 2.5, 1.3)])
-def test_fp32_metropolis_tracks_fp64(u1, orc, proposal, nhit, step, beta):
-    """The binary32 hit (proposal FMA, wrap, 2^(beta log2(e) dRe + 24) > u24)
-    reads the same Philox words as the binary64 hit, which matches the oracle
-    bit for bit.  From a configuration exactly representable in FP32, one
-    Metropolis pass in each precision: the accept decisions agree except for
-    the rare draws within rounding of the threshold (and the neighbours they
-    perturb later in the pass), and the angles agree to FP32 rounding."""
+@pytest.mark.parametrize("proposal,nhit,step,beta,aniso", [(0, 1, 0.0, 1.0, 1.0), (0, 2, 0.0, 0.7, 1.0),
+                                                           (1, 4, 0.5, 1.0, 1.0), (1, 3, 2.5, 1.3, 1.0),
+                                                           (1, 4, 0.5, 1.0, 2.0)])
+def test_fp32_metropolis_tracks_fp64(u1, orc, proposal, nhit, step, beta, aniso):
+    """The binary32 hit (proposal FMA, 2^(beta log2(e) dRe + 32) > raw word,
+    binary64 wrap per pair of hits) reads the same Philox words as the
+    binary64 hit, which matches the oracle bit for bit.  From a configuration
+    exactly representable in FP32, one Metropolis pass in each precision: the
+    accept decisions agree except for the rare draws within rounding of the
+    threshold (and the neighbours they perturb later in the pass), and the
+    angles agree to FP32 rounding."""
     dims = (8, 8, 8, 8)
-    g = u1.geom(dims, beta)
-    og = orc.geom(dims, beta)
+    g = u1.geom(dims, beta, aniso)
+    og = orc.geom(dims, beta, aniso)
     ref = orc.hotstart(og, 17)
     for s in range(5):
         orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 17, s)
@@ -440,7 +445,7 @@ def test_fp32_metropolis_tracks_fp64(u1, orc, proposal, nhit, step, beta):
     d = np.abs(np.angle(np.exp(1j * (a64.to_host() - a32.to_host().astype(np.float64)))))
     links = d.size
     moved = np.mean(np.abs(np.angle(np.exp(1j * (a64.to_host() - start)))) > 1e-6)
-    say(f"fp32 Metropolis vs fp64: proposal {proposal} nhit {nhit} step {step} beta {beta}: accepts "
+    say(f"fp32 Metropolis vs fp64: proposal {proposal} nhit {nhit} step {step} beta {beta} xi {aniso}: accepts "
         f"{n32} vs {n64} of {links * nhit} hits, links moved {moved:.3f}, links off by > 1e-4: {np.sum(d > 1e-4)}, "
         f"median |d| {np.median(d):.1e}")
     assert moved > 0.2
