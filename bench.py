@@ -127,6 +127,9 @@ def main():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
                     help="counter_collection.csv files of earlier rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs "
                          "of this script (with --calib-mb): fills roofline.traffic")
+    ap.add_argument("--pair-fusion", type=int, choices=[0, 1, 2], default=1,
+                    help="fused (parity, 0) + (parity, 1) update launches: 0 never, 1 where the tuner measures "
+                         "them faster, 2 always")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the run, copy a buffer of this many MiB once (plain copy kernel of known size, "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -145,6 +148,7 @@ def main():
     dev = u1.device_name()
     if args.verbose:
         u1.set_verbosity(2)
+    u1.set_pair_fusion(args.pair_fusion)
     if args.tunecache and os.path.exists(args.tunecache):
         u1.tune_load(args.tunecache)
 

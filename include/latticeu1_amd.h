@@ -196,6 +196,14 @@ int u1_ensemble_destroy(u1_ensemble* e);
 /* Tuning picks the workgroup size per (kernel, geometry, precision) by timing
  * candidates on a scratch copy (results are independent of the choice). */
 int u1_tune_enable(int32_t on);
+/* Fused update launches (no reference counterpart): on a single lattice the
+ * phases (parity, 0) and (parity, 1) of a pass can run as one launch, one
+ * workgroup per (z, t) plane with a workgroup barrier between the phases; the
+ * results are bit-identical to two launches.  mode 0: never; 1 (default):
+ * where the tuner measures the fused launch faster (needs tuning on); 2:
+ * always.  The tune cache keeps the decision as "MetropolisPair" /
+ * "OverRelaxationPair" entries (block.x 0 = two launches). */
+int u1_set_pair_fusion(int32_t mode);
 /* Verbosity (parameters.h:33-37): 0 SILENT, 1 SUMMARIZE (the tuner reports
  * its choice on stderr), 2 VERBOSE and 3 DEBUG_VERBOSE (every candidate's
  * time).  Callable before u1_init. */

@@ -109,6 +109,7 @@ _SIGS = {
     "u1_ensemble_info": (_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int64)),
     "u1_ensemble_destroy": (_P,),
     "u1_tune_enable": (C.c_int32,),
+    "u1_set_pair_fusion": (C.c_int32,),
     "u1_set_verbosity": (C.c_int32,),
     "u1_tune_load": (C.c_char_p,),
     "u1_tune_save": (C.c_char_p,),
@@ -450,6 +451,12 @@ def set_verbosity(level: int) -> None:
 
 def tune_enable(on: bool) -> None:
     call("u1_tune_enable", 1 if on else 0)
+
+
+def set_pair_fusion(mode: int) -> None:
+    """Fused (parity, 0) + (parity, 1) update launches on single lattices:
+    0 never, 1 where the tuner measures them faster (default), 2 always."""
+    call("u1_set_pair_fusion", mode)
 
 
 def tune_save(path: str) -> None:

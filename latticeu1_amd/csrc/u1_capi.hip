@@ -39,6 +39,9 @@ struct Ctx {
   unsigned long long* counter = nullptr;
   double* dres = nullptr;  // small device result buffer (64 doubles)
   bool tune = true;
+  // fused (parity, 0) + (parity, 1) update launches on single lattices:
+  // 0 never, 1 where the tuner measures them faster, 2 always
+  int pair = 1;
   int verbosity = 0;                      // 0 silent, 1 summarize, 2 verbose (tuner output on stderr)
   std::map<std::string, Cfg> tunecache;  // key -> launch shape
   int tune_loaded = 0;
@@ -325,6 +328,145 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
   return U1_OK;
 }
 
+// Fused pair shapes of a geometry: whole (z, t) planes per workgroup, SPT
+// sites per thread per step, the workgroup size rounded up to whole
+// wavefronts and capped at the kernel's bound (larger planes take more steps).
+std::vector<Cfg> pair_candidates(int prec, const RepDesc& r) {
+  const int P = r.nxh * r.N[1];
+  const int maxb = prec == U1_F64 ? 512 : 1024;
+  std::vector<Cfg> v;
+  for (int spt : {1, 2, 4}) {
+    if (spt == 4 && prec != U1_F32) continue;
+    const int b = std::min(maxb, ((P + spt - 1) / spt + 63) / 64 * 64);
+    for (int t = 0; t < 2; ++t)
+      for (int x = 0; x < 2; ++x) v.push_back(Cfg{b, spt, x, t});
+  }
+  return v;
+}
+bool pair_shape_ok(int prec, const Cfg& c) {
+  const int maxb = prec == U1_F64 ? 512 : 1024;
+  return c.block >= 64 && c.block <= maxb && c.block % 64 == 0 &&
+         (c.spt == 1 || c.spt == 2 || (c.spt == 4 && prec == U1_F32));
+}
+// Plane decomposition of the fused pair: one chunk (workgroup) per plane.
+void set_pair_blocks(RepDesc& r, const Cfg& c) {
+  u1d::set_phase_blocks(r, r.nxh * r.N[1]);
+  u1d::set_phase_tiles(r, c.tile != 0);
+}
+
+int time_pair(int prec, int kind, void* lat, const PhaseArgs& a, const Cfg& c, float* ms) {
+  EventSet es;
+  hipEvent_t e0, e1;
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
+  HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, G.st));  // warm-up
+  HIPCHK(hipEventRecord(e0, G.st));
+  for (int rep = 0; rep < 4; ++rep)
+    HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, G.st));
+  HIPCHK(hipEventRecord(e1, G.st));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  return U1_OK;
+}
+
+// Shape of the fused (parity, 0) + (parity, 1) launch for kernel class `kind`
+// on this geometry, or block = 0 for two separate launches.  With tuning, the
+// best pair shape is timed against the two phases in the single-phase shape
+// `single` (tuned), on a scratch copy; the decision is cached with the shapes.
+int pair_cfg(int prec, int kind, const u1_geom* g, const void* lat, const PhaseArgs& proto, const Cfg& single,
+             Cfg* cfg) {
+  *cfg = Cfg{0, 1, 0, 0};
+  if (G.pair == 0) return U1_OK;
+  const char* name = kind == u1d::KIND_METROPOLIS ? "MetropolisPair" : "OverRelaxationPair";
+  const std::string key = tune_key(name, g, prec);
+  const std::vector<Cfg> cands = pair_candidates(prec, proto.single);
+  // forced without a tuned pair shape: XCD order, lexicographic planes, 2
+  // (binary32) or 1 (binary64) sites per thread
+  auto forced_default = [&]() {
+    for (const Cfg& c : cands)
+      if (c.spt == (prec == U1_F32 ? 2 : 1) && c.xcd == 1 && c.tile == 0) *cfg = c;
+  };
+  auto it = G.tunecache.find(key);
+  if (it != G.tunecache.end()) {  // block 0: the tuner measured two launches faster
+    if (it->second.block != 0) *cfg = it->second;
+    else if (G.pair == 2) forced_default();
+    return U1_OK;
+  }
+  if (!G.tune) {
+    if (G.pair == 2) forced_default();
+    return U1_OK;
+  }
+  const int64_t bytes = 4 * (int64_t)proto.single.V * (int64_t)elem_size(prec);
+  DevBuf scratch;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(hipMemcpyAsync(scratch.p, lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  std::vector<float> tbest(cands.size(), 1e30f);
+  float tsep = 1e30f;  // phases (parity, 0) and (parity, 1) as two launches each
+  for (int round = 0; round < 3; ++round) {
+    for (size_t i = 0; i < cands.size(); ++i) {
+      PhaseArgs a = proto;
+      a.accepts = nullptr;
+      a.meas = nullptr;
+      a.xcd = (uint32_t)cands[i].xcd;
+      set_pair_blocks(a.single, cands[i]);
+      float ms = 0;
+      RET(time_pair(prec, kind, scratch.p, a, cands[i], &ms));
+      tbest[i] = std::min(tbest[i], ms);
+    }
+    PhaseArgs a = proto;
+    a.accepts = nullptr;
+    a.meas = nullptr;
+    a.xcd = (uint32_t)single.xcd;
+    set_update_blocks(a.single, single);
+    EventSet es;
+    hipEvent_t e0, e1;
+    HIPCHK(es.create(&e0));
+    HIPCHK(es.create(&e1));
+    HIPCHK(hipEventRecord(e0, G.st));
+    for (int rep = 0; rep < 4; ++rep)
+      for (int mu = 0; mu < 2; ++mu)
+        HIPCHK(u1d::launch_phase(prec, kind, mu, scratch.p, a, a.single.nblk, single.block, single.spt, G.st));
+    HIPCHK(hipEventRecord(e1, G.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    tsep = std::min(tsep, ms);
+  }
+  size_t ib = 0;
+  for (size_t i = 1; i < cands.size(); ++i)
+    if (tbest[i] < tbest[ib]) ib = i;
+  const bool use = G.pair == 2 || tbest[ib] < tsep;
+  if (G.verbosity >= 2)
+    for (size_t i = 0; i < cands.size(); ++i)
+      std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %s order%s: %.2f us per pair launch%s\n",
+                   key.c_str(), cands[i].block, cands[i].spt, cands[i].xcd ? "XCD" : "plain",
+                   cands[i].tile ? ", tiled planes" : "", 1e3 * tbest[i] / 4.0, i == ib ? "  <- best" : "");
+  if (G.verbosity >= 1)
+    std::fprintf(stderr, "tune %s: pair %.2f us (block %d, %d site(s)/thread) vs two launches %.2f us: %s\n",
+                 key.c_str(), 1e3 * tbest[ib] / 4.0, cands[ib].block, cands[ib].spt, 1e3 * tsep / 4.0,
+                 use ? "fused" : "separate");
+  const Cfg chosen = use ? cands[ib] : Cfg{0, 1, 0, 0};
+  G.tunecache[key] = tbest[ib] < tsep ? cands[ib] : Cfg{0, 1, 0, 0};
+  *cfg = chosen;
+  return U1_OK;
+}
+
+// Settles (tunes, if enabled) every launch shape a sweep of this geometry
+// uses, so that a later sweep allocates and synchronises nothing (graph
+// capture, timed regions).
+int settle_update_cfgs(int prec, const u1_geom* g, const void* lat, const PhaseArgs& proto, int metrop, int ovrn) {
+  Cfg c, pc;
+  if (metrop > 0) {
+    RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, proto, &c));
+    RET(pair_cfg(prec, u1d::KIND_METROPOLIS, g, lat, proto, c, &pc));
+  }
+  if (ovrn > 0) {
+    RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, proto, &c));
+    RET(pair_cfg(prec, u1d::KIND_OVERRELAX, g, lat, proto, c, &pc));
+  }
+  return U1_OK;
+}
+
 // Events around one sweep's launch groups: start | Metropolis | after_metro |
 // OR phases without measurement | before_meas | measuring phases (and the OR
 // phases between them) | after_or.  Without measurement before_meas is
@@ -358,52 +500,66 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   a.sweep_dev = sweep_dev;
   a.sweep_mul = sweep_mul;
   a.accepts = acc_dev;
-  Cfg cm = kDefaultCfg, co = kDefaultCfg;
+  Cfg cm = kDefaultCfg, co = kDefaultCfg, pm{0, 1, 0, 0}, po{0, 1, 0, 0};
   if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
   if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &co));
-  RepDesc rm = a.single, ro = a.single;
+  if (metrop > 0) RET(pair_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, cm, &pm));
+  if (ovrn > 0) RET(pair_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, co, &po));
+  RepDesc rm = a.single, ro = a.single, rpm = a.single, rpo = a.single;
   set_update_blocks(rm, cm);
   set_update_blocks(ro, co);
+  set_pair_blocks(rpm, pm);
+  set_pair_blocks(rpo, po);
   if (meas_dst && (ml != 0 || metrop + ovrn == 0)) return fail(U1_ERR_ARG, "no measuring pass");
-  const int meas_nblk = ovrn > 0 ? ro.nblk : rm.nblk;  // rows per measuring phase
-  if (meas_dst) RET(ensure_fold(3 * meas_nblk, 4));
-  // measuring phases: parity 1, mu >= 1 of the last pass
+  // measuring phases: parity 1, mu >= 1 of the last pass; rows of phase 1 (in
+  // a fused pair: one per plane), then of phases 2 and 3
+  const bool meas_or = ovrn > 0;
+  const int meas_nblk = meas_or ? ro.nblk : rm.nblk;
+  const int meas_nblk1 = (meas_or ? po.block : pm.block) != 0 ? (meas_or ? rpo.nblk : rpm.nblk) : meas_nblk;
+  const int meas_rows_total = meas_nblk1 + 2 * meas_nblk;
+  if (meas_dst) RET(ensure_fold(meas_rows_total, 4));
   auto meas_rows = [&](bool last_pass, int parity, int mu) -> double* {
     if (!meas_dst || !last_pass || parity != 1 || mu < 1) return nullptr;
-    return G.partials + 4 * (size_t)meas_nblk * (size_t)(mu - 1);
+    return G.partials + 4 * (mu == 1 ? 0 : (size_t)meas_nblk1 + (size_t)meas_nblk * (size_t)(mu - 2));
+  };
+  // one pass of kind `kind`: per parity, (parity, 0) + (parity, 1) fused or
+  // not, then (parity, 2) and (parity, 3)
+  auto pass = [&](int kind, uint32_t p, bool last, const Cfg& c, const RepDesc& r, const Cfg& pc,
+                  const RepDesc& rp, bool* marked) -> int {
+    a.pass = p;
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.parity = parity;
+        a.meas = meas_rows(last, parity, mu == 0 && pc.block != 0 ? 1 : mu);
+        if (gev && marked && a.meas && !*marked) {
+          HIPCHK(hipEventRecord(gev->before_meas, G.st));
+          *marked = true;
+        }
+        if (mu == 0 && pc.block != 0) {
+          a.single = rp;
+          a.xcd = (uint32_t)pc.xcd;
+          HIPCHK(u1d::launch_pair(prec, kind, lat, a, rp.nblk, pc.block, pc.spt, G.st));
+          ++mu;
+          continue;
+        }
+        a.single = r;
+        a.xcd = (uint32_t)c.xcd;
+        HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, r.nblk, c.block, c.spt, G.st));
+      }
+    return U1_OK;
   };
   if (gev) HIPCHK(hipEventRecord(gev->start, G.st));
-  a.single = rm;
-  a.xcd = (uint32_t)cm.xcd;
   for (int p = 0; p < metrop; ++p)
-    for (int parity = 0; parity < 2; ++parity)
-      for (int mu = 0; mu < 4; ++mu) {
-        a.pass = (uint32_t)p;
-        a.parity = parity;
-        a.meas = meas_rows(ovrn == 0 && p == metrop - 1, parity, mu);
-        HIPCHK(u1d::launch_phase(prec, u1d::KIND_METROPOLIS, mu, lat, a, rm.nblk, cm.block, cm.spt, G.st));
-      }
+    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, ovrn == 0 && p == metrop - 1, cm, rm, pm, rpm, nullptr));
   if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
-  a.single = ro;
-  a.xcd = (uint32_t)co.xcd;
   bool marked = false;
   for (int o = 0; o < ovrn; ++o)
-    for (int parity = 0; parity < 2; ++parity)
-      for (int mu = 0; mu < 4; ++mu) {
-        a.pass = (uint32_t)o;
-        a.parity = parity;
-        a.meas = meas_rows(o == ovrn - 1, parity, mu);
-        if (gev && a.meas && !marked) {
-          HIPCHK(hipEventRecord(gev->before_meas, G.st));
-          marked = true;
-        }
-        HIPCHK(u1d::launch_phase(prec, u1d::KIND_OVERRELAX, mu, lat, a, ro.nblk, co.block, co.spt, G.st));
-      }
+    RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == ovrn - 1, co, ro, po, rpo, &marked));
   if (gev && !marked) HIPCHK(hipEventRecord(gev->before_meas, G.st));
   if (gev) HIPCHK(hipEventRecord(gev->after_or, G.st));
   if (meas_dst)
-    HIPCHK(u1d::launch_finish_sums(G.partials, 3 * meas_nblk, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
-                                   fold_scratch_ptr(3 * meas_nblk, 4), G.st));
+    HIPCHK(u1d::launch_finish_sums(G.partials, meas_rows_total, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
+                                   fold_scratch_ptr(meas_rows_total, 4), G.st));
   return U1_OK;
 }
 
@@ -927,9 +1083,7 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
     a.nhit = m->nhit;
     a.proposal = m->proposal;
     a.step = m->step;
-    Cfg c;
-    if (metrop > 0) rc = phase_cfg(prec, u1d::KIND_METROPOLIS, g, wl, a, &c);
-    if (rc == U1_OK && ovrn > 0) rc = phase_cfg(prec, u1d::KIND_OVERRELAX, g, wl, a, &c);
+    rc = settle_update_cfgs(prec, g, wl, a, metrop, ovrn);
     if (rc != U1_OK) {
       cleanup();
       return rc;
@@ -1311,6 +1465,12 @@ int u1_tune_enable(int32_t on) {
   return U1_OK;
 }
 
+int u1_set_pair_fusion(int32_t mode) {
+  if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "pair fusion mode must be 0, 1 or 2");
+  G.pair = mode;
+  return U1_OK;
+}
+
 int u1_set_verbosity(int32_t level) {
   if (level < 0 || level > 3) return fail(U1_ERR_ARG, "verbosity must be in [0, 3]");
   G.verbosity = level;
@@ -1350,7 +1510,10 @@ int u1_tune_load(const char* path) {
     // whole wavefronts, at most 512 threads and 4 sites per thread (binary32)
     const bool f32 = name.size() > 4 && name.compare(name.size() - 4, 4, "_f32") == 0;
     bool known = false;
-    for (const Cfg& c : kShapes) known = known || (c.block == bx && c.spt == by);
+    if (name.find("Pair_") != std::string::npos)  // fused pair: block 0 = two launches
+      known = (bx == 0 && by == 1) || pair_shape_ok(f32 ? U1_F32 : U1_F64, Cfg{bx, by, 0, 0});
+    else
+      for (const Cfg& c : kShapes) known = known || (c.block == bx && c.spt == by);
     if (!known || (by == 4 && !f32) || bz < 1 || bz > 4)
       return fail(U1_ERR_IO, std::string("Bad launch shape in ") + path + ": " + line);
     parsed.push_back({vol + "\t" + name, Cfg{bx, by, (bz - 1) & 1, (bz - 1) >> 1}});
@@ -1373,7 +1536,7 @@ int u1_tune_save(const char* path) {
     f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.block << "\t" << kv.second.spt << "\t"
       << 1 + kv.second.xcd + 2 * kv.second.tile
       << "\t0\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z - 1 = XCD-aware order (1) + "
-         "tiled planes (2)\n";
+         "tiled planes (2); Pair: fused (parity, 0) + (parity, 1) launch, block.x 0 = two launches\n";
   }
   return f ? U1_OK : fail(U1_ERR_IO, "write failed");
 }
@@ -1406,9 +1569,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
     a.nhit = m->nhit;
     a.proposal = m->proposal;
     a.step = m->step;
-    Cfg c;
-    if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &c));
-    if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &c));
+    RET(settle_update_cfgs(prec, g, lat, a, metrop, ovrn));
   }
   // Timed region: every sweep brackets its Metropolis and OR launch groups
   // with events (kernel_ms), plus one event pair around the whole region.

@@ -38,6 +38,13 @@ enum { PREC_F64 = 0, PREC_F32 = 1 };
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
                         int spt, hipStream_t st);
 
+// The phases (parity, 0) and (parity, 1) of a single lattice in one launch
+// (k_pair): a.single must carry set_phase_blocks(r, Nx/2 * Ny), one workgroup
+// per (z, t) plane (nblocks = Nz * Nt); block a multiple of 64 up to 1024
+// (binary32) or 512 (binary64).  a.meas makes phase (parity, 1) measuring.
+hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
+                       hipStream_t st);
+
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t st);
 
 // measurement launchers (u1_measure.hip)

@@ -52,27 +52,60 @@ __host__ __device__ constexpr int first_nu(int mu, bool st) {
   return -1;
 }
 
-// Staple of link (x, MU) (staple.h:79-102), with the operation order of the
-// oracle.  fwd/bwd[d] are the eo-id offsets to x +- e_d.  With MEAS the
-// plaquette pieces are collected on the side (the staple is unchanged).
-template <typename Real, int MU, bool MEAS = false>
-__device__ __forceinline__ Staple<Real> staple_at(const Real* __restrict__ lat, const RepDesc& r, uint32_t same,
-                                                  uint32_t opp, const int32_t (&fwd)[4], const int32_t (&bwd)[4],
-                                                  Real xi, Real inv_xi, bool aniso, PlaqPieces<Real>* pp = nullptr) {
-  Real ssr = 0, ssi = 0, str = 0, sti = 0;
+// The 18 angles of the staple of link (x, MU) (staple.h:85-92): for the j-th
+// direction nu != MU (j = nu for nu < MU, else nu - 1),
+//   a = theta_nu(x+mu), b = theta_mu(x+nu), c = theta_nu(x),
+//   d = theta_nu(x+mu-nu), e = theta_mu(x-nu), f = theta_nu(x-nu).
+// fwd/bwd[d] are the eo-id offsets to x +- e_d.
+template <typename Real>
+struct StapleIn {
+  Real a[3], b[3], c[3], d[3], e[3], f[3];
+};
+
+// Loads the staple angles.  With DEFER0 the two of them that read theta_0 at
+// the link's own parity (c and d of nu = 0) are left to staple_load_nu0.
+template <typename Real, int MU, bool DEFER0 = false>
+__device__ __forceinline__ void staple_load(const Real* __restrict__ lat, const RepDesc& r, uint32_t same,
+                                            uint32_t opp, const int32_t (&fwd)[4], const int32_t (&bwd)[4],
+                                            StapleIn<Real>& in) {
   const Real* pm = lat + (size_t)MU * r.V;
 #pragma unroll
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == MU) continue;
+    const int j = nu < MU ? nu : nu - 1;
     const Real* pn = lat + (size_t)nu * r.V;
-    const Real a = ld(pn, opp + fwd[MU]);             // theta_nu(x+mu)
-    const Real b = ld(pm, opp + fwd[nu]);             // theta_mu(x+nu)
-    const Real c = ld(pn, same);                      // theta_nu(x)
-    const Real d = ld(pn, same + fwd[MU] + bwd[nu]);  // theta_nu(x+mu-nu)
-    const Real e = ld(pm, opp + bwd[nu]);             // theta_mu(x-nu)
-    const Real f = ld(pn, opp + bwd[nu]);             // theta_nu(x-nu)
-    const Real up = (a - b) - c;
-    const Real lo = ((-d) - e) + f;
+    in.a[j] = ld(pn, opp + fwd[MU]);
+    in.b[j] = ld(pm, opp + fwd[nu]);
+    if (!(DEFER0 && nu == 0)) {
+      in.c[j] = ld(pn, same);
+      in.d[j] = ld(pn, same + fwd[MU] + bwd[nu]);
+    }
+    in.e[j] = ld(pm, opp + bwd[nu]);
+    in.f[j] = ld(pn, opp + bwd[nu]);
+  }
+}
+template <typename Real, int MU>
+__device__ __forceinline__ void staple_load_nu0(const Real* __restrict__ lat, uint32_t same,
+                                                const int32_t (&fwd)[4], const int32_t (&bwd)[4],
+                                                StapleIn<Real>& in) {
+  static_assert(MU != 0, "nu = 0 is a staple direction only for MU != 0");
+  in.c[0] = ld(lat, same);
+  in.d[0] = ld(lat, same + fwd[MU] + bwd[0]);
+}
+
+// Staple of link (x, MU) (staple.h:79-102) from its angles, with the operation
+// order of the oracle.  With MEAS the plaquette pieces are collected on the
+// side (the staple is unchanged).
+template <typename Real, int MU, bool MEAS = false>
+__device__ __forceinline__ Staple<Real> staple_sum(const StapleIn<Real>& in, Real xi, Real inv_xi, bool aniso,
+                                                   PlaqPieces<Real>* pp = nullptr) {
+  Real ssr = 0, ssi = 0, str = 0, sti = 0;
+#pragma unroll
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == MU) continue;
+    const int j = nu < MU ? nu : nu - 1;
+    const Real up = (in.a[j] - in.b[j]) - in.c[j];
+    const Real lo = ((-in.d[j]) - in.e[j]) + in.f[j];
     Real su, cu, sl, cl;
     UMath<Real>::sincos(up, &su, &cu);
     UMath<Real>::sincos(lo, &sl, &cl);
@@ -179,37 +212,73 @@ __device__ __forceinline__ bool metro_hit_f32(MetroState<float>& m, Staple<float
 // -sin P] in the space-space (MU < 3) or space-time (MU = 3) slot.  The three
 // phases (1, 1), (1, 2), (1, 3) come after every link of those planes has
 // reached its final value, and together cover all six planes once.
-template <typename Real, int MU, int KIND, bool BATCHED, int SPT, bool MEAS = false>
-__global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, PhaseArgs a) {
-  const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
-  const uint32_t b = L - (uint32_t)r.blk_begin;
+//
+// A thread's share of a phase is the sites tid0 + k blockDim.x (k < SPT) of
+// chunk b in phase_site numbering (bdim sites per chunk), from phase_sites.
+// phase_load issues all of their angle loads; phase_compute forms the staples, updates and
+// stores the links, adds the plaquette sums of its valid sites to v (MEAS) and
+// the accepted hits of its wavefront to acc_wave; the caller reduces both.
+template <typename Real, int SPT>
+struct PhaseIn {
   PhaseSite ps[SPT];
+  uint32_t same[SPT];
+  StapleIn<Real> in[SPT];
+  Real th[SPT];
+};
+
+// The thread's sites; returns true if the workgroup's plane is frozen
+// (multilevel.cu:41,141 -- spatial links on slices t % ml == 0 are frozen; t
+// is uniform over the workgroup, so the whole workgroup skips the phase).
+template <int MU, int SPT>
+__device__ __forceinline__ bool phase_sites(const PhaseArgs& a, const RepDesc& r, uint32_t b, uint32_t tid0,
+                                            uint32_t bdim, PhaseSite (&ps)[SPT]) {
 #pragma unroll
-  for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, a.parity, ps[k]);
-  // multilevel.cu:41,141 -- spatial links on slices t % ml == 0 are frozen;
-  // t is uniform over the workgroup, so the whole workgroup leaves.
-  if (MU != 3 && a.ml != 0 && (ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
-  Real* lat = lat_base + r.lat_off;
-  Real* pmu = lat + (size_t)MU * r.V;
+  for (int k = 0; k < SPT; ++k) phase_site(r, b, tid0 + k * blockDim.x, bdim, a.parity, ps[k]);
+  return MU != 3 && a.ml != 0 && (ps[0].t & (uint32_t)(a.ml - 1)) == 0;
+}
+
+// The angle loads of the sites in q.ps.
+template <typename Real, int MU, int SPT, bool DEFER0 = false>
+__device__ __forceinline__ void phase_load(const Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
+                                           PhaseIn<Real, SPT>& q) {
+  const Real* lat = lat_base + r.lat_off;
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    q.same[k] = (uint32_t)(a.parity * r.HV) + q.ps[k].id;
+    const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + q.ps[k].id;
+    staple_load<Real, MU, DEFER0>(lat, r, q.same[k], opp, q.ps[k].fwd, q.ps[k].bwd, q.in[k]);
+    q.th[k] = ld(lat + (size_t)MU * r.V, q.same[k]);
+  }
+}
+
+// The loads phase_load<..., DEFER0 = true> left out.
+template <typename Real, int MU, int SPT>
+__device__ __forceinline__ void phase_load_deferred(const Real* __restrict__ lat_base, const RepDesc& r,
+                                                    PhaseIn<Real, SPT>& q) {
+#pragma unroll
+  for (int k = 0; k < SPT; ++k)
+    staple_load_nu0<Real, MU>(lat_base + r.lat_off, q.same[k], q.ps[k].fwd, q.ps[k].bwd, q.in[k]);
+}
+
+template <typename Real, int MU, int KIND, int SPT, bool MEAS>
+__device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
+                                              PhaseIn<Real, SPT>& q, double (&v)[2], uint32_t& acc_wave) {
+  Real* pmu = lat_base + r.lat_off + (size_t)MU * r.V;
   const bool aniso = r.aniso != 1.0;
   const Real xi = sizeof(Real) == sizeof(float) ? (Real)r.aniso_f : (Real)r.aniso;
-  uint32_t same[SPT];
+  const PhaseSite(&ps)[SPT] = q.ps;
+  const uint32_t(&same)[SPT] = q.same;
   Staple<Real> S[SPT];
   Real th[SPT];
   PlaqPieces<Real> pp[SPT];
 #pragma unroll
   for (int k = 0; k < SPT; ++k) {
-    same[k] = (uint32_t)(a.parity * r.HV) + ps[k].id;
-    const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + ps[k].id;
     pp[k] = {0, 0, 0, 0};
-    S[k] = staple_at<Real, MU, MEAS>(lat, r, same[k], opp, ps[k].fwd, ps[k].bwd, xi, (Real)r.inv_aniso_f, aniso,
-                                     &pp[k]);
-    th[k] = ld(pmu, same[k]);
+    S[k] = staple_sum<Real, MU, MEAS>(q.in[k], xi, (Real)r.inv_aniso_f, aniso, &pp[k]);
+    th[k] = q.th[k];
   }
   // accepted hits of this wavefront (valid sites only): a wave-uniform count
   // from the accept masks, kept on the scalar unit
-  uint32_t acc_wave = 0;
   if (KIND == KIND_METROPOLIS) {
     MetroState<Real> m[SPT];
     uint32_t l[SPT];
@@ -271,7 +340,6 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   for (int k = 0; k < SPT; ++k)
     if (ps[k].valid) st(pmu, same[k], th[k]);
   if (MEAS) {
-    double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
       Real sn, cn;
@@ -284,6 +352,14 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
         v[1] += (double)im;
       }
     }
+  }
+}
+
+// Workgroup epilogue: the plaquette row L of a measuring phase (block_sum:
+// every thread of the workgroup takes part) and the wavefront's accepts.
+template <int MU, int KIND, bool MEAS>
+__device__ __forceinline__ void phase_finish(const PhaseArgs& a, uint32_t L, double (&v)[2], uint32_t acc_wave) {
+  if (MEAS) {
     double o[2];
     block_sum<2>(v, o);
     if (threadIdx.x == 0) {
@@ -297,6 +373,83 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
   }
   if (KIND == KIND_METROPOLIS && a.accepts && (threadIdx.x & 63) == 0 && acc_wave)
     atomicAdd(a.accepts, (unsigned long long)acc_wave);
+}
+
+template <typename Real, int MU, int KIND, bool BATCHED, int SPT, bool MEAS = false>
+__global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, PhaseArgs a) {
+  const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
+  const uint32_t b = L - (uint32_t)r.blk_begin;
+  double v[2] = {0.0, 0.0};
+  uint32_t acc_wave = 0;
+  PhaseIn<Real, SPT> q;
+  // (phase_sites and the early exit written out here: through the helper,
+  // hipcc turns the site arithmetic into divergent branches)
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, SPT * blockDim.x, a.parity, q.ps[k]);
+  if (MU != 3 && a.ml != 0 && (q.ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
+  phase_load<Real, MU, SPT>(lat_base, a, r, q);
+  phase_compute<Real, MU, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc_wave);
+  phase_finish<MU, KIND, MEAS>(a, L, v, acc_wave);
+}
+
+// The phases (parity, 0) and (parity, 1) of one single lattice in one launch.
+// Workgroup L owns the whole (z, t) plane L (set_phase_blocks(r, P): one chunk
+// per plane) and runs phase (parity, 0) over it, SPT * blockDim.x sites per
+// step, then phase (parity, 1), with a workgroup barrier in between.  This is
+// race free and gives the results of the two separate launches bit for bit:
+//  * the staple of (x, 1) reads the phase-0 links theta_0 at this parity only
+//    at x and x + e1 - e0, both in the same (z, t) plane, so in this
+//    workgroup, and written before the barrier;
+//  * phase (parity, 0) reads theta_1 at this parity only at x and x + e0 - e1,
+//    also in the same plane, so before this workgroup's phase 1 rewrites them;
+//  * nothing else either phase reads is written by the launch.
+// __syncthreads orders the workgroup's global stores before its later loads.
+// The pair reads the lattice once (4V angles) and writes V/2 + V/2: 5V angles
+// against 9V for two launches.  MEAS applies to phase (parity, 1).
+// When one step covers the plane (P <= SPT * blockDim.x, SPT <= 2), every
+// load of both phases except the two phase-0 results is issued before phase 0
+// computes, so phase 1 waits on the barrier and two cache-hit loads only.
+template <typename Real, int KIND, int SPT, bool MEAS>
+__global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_pair(Real* __restrict__ lat_base,
+                                                                                     PhaseArgs a) {
+  const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const RepDesc& r = a.single;
+  const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
+  const uint32_t step = SPT * blockDim.x;
+  double v0[2] = {0.0, 0.0}, v[2] = {0.0, 0.0};
+  uint32_t acc0 = 0, acc1 = 0;
+  if (SPT <= 2 && P <= step) {
+    PhaseIn<Real, SPT> q0, q1;
+    // a frozen plane (multilevel mask, uniform over the workgroup) is frozen
+    // for both phases: the whole workgroup leaves before the barrier
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) phase_site(r, L, threadIdx.x + k * blockDim.x, step, a.parity, q0.ps[k]);
+    if (a.ml != 0 && (q0.ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) q1.ps[k] = q0.ps[k];  // the same sites
+    phase_load<Real, 0, SPT>(lat_base, a, r, q0);
+    phase_load<Real, 1, SPT, true>(lat_base, a, r, q1);
+    phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q0, v0, acc0);
+    __syncthreads();
+    phase_load_deferred<Real, 1, SPT>(lat_base, r, q1);
+    phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q1, v, acc1);
+  } else {
+    for (uint32_t base = 0; base < P; base += step) {
+      PhaseIn<Real, SPT> q;
+      if (phase_sites<0, SPT>(a, r, L, base + threadIdx.x, step, q.ps)) return;  // frozen plane
+      phase_load<Real, 0, SPT>(lat_base, a, r, q);
+      phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q, v0, acc0);
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < P; base += step) {
+      PhaseIn<Real, SPT> q;
+      phase_sites<1, SPT>(a, r, L, base + threadIdx.x, step, q.ps);
+      phase_load<Real, 1, SPT>(lat_base, a, r, q);
+      phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
+    }
+  }
+  phase_finish<1, KIND, MEAS>(a, L, v, acc0 + acc1);
 }
 
 template <typename Real>
@@ -350,19 +503,58 @@ hipError_t launch_prec(int kind, int mu, Real* lat, const PhaseArgs& a, int nblo
                  : launch_spt<Real, KIND_OVERRELAX, false>(spt, mu, lat, a, nblocks, block, s);
 }
 
-}  // namespace
-
-hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
-                        int spt, hipStream_t s) {
-  if (prec == PREC_F64) return launch_prec<double>(kind, mu, (double*)lat_base, a, nblocks, block, spt, s);
-  // binary32 proposal constants: a (2 wu 2^-32 - 1) with a = pi (independent
-  // angle) or step (theta + delta); a 2^-31 is exact
+// binary32 proposal constants: a (2 wu 2^-32 - 1) with a = pi (independent
+// angle) or step (theta + delta); a 2^-31 is exact
+PhaseArgs with_f32_constants(const PhaseArgs& a) {
   PhaseArgs b = a;
   const float amp = a.proposal == 0 ? u1m::KF_PI : (float)a.step;
   b.pscale_f = amp * 0x1p-31f;
   b.poff_f = amp;
   b.thm_f = a.proposal == 0 ? 0.0f : 1.0f;
-  return launch_prec<float>(kind, mu, (float*)lat_base, b, nblocks, block, spt, s);
+  return b;
+}
+
+template <typename Real, int KIND, int SPT>
+hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+  if (a.meas != nullptr)
+    k_pair<Real, KIND, SPT, true><<<nblocks, block, 0, s>>>(lat, a);
+  else
+    k_pair<Real, KIND, SPT, false><<<nblocks, block, 0, s>>>(lat, a);
+  return hipGetLastError();
+}
+
+template <typename Real, int KIND>
+hipError_t launch_pair_spt(int spt, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+  if (spt == 2) return launch_pair_meas<Real, KIND, 2>(lat, a, nblocks, block, s);
+  if (spt == 4) return launch_pair_meas<Real, KIND, (sizeof(Real) == sizeof(float) ? 4 : 1)>(lat, a, nblocks, block, s);
+  return launch_pair_meas<Real, KIND, 1>(lat, a, nblocks, block, s);
+}
+
+template <typename Real>
+hipError_t launch_pair_prec(int kind, Real* lat, const PhaseArgs& a, int nblocks, int block, int spt,
+                            hipStream_t s) {
+  if (kind == KIND_METROPOLIS) return launch_pair_spt<Real, KIND_METROPOLIS>(spt, lat, a, nblocks, block, s);
+  return launch_pair_spt<Real, KIND_OVERRELAX>(spt, lat, a, nblocks, block, s);
+}
+
+}  // namespace
+
+hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
+                        int spt, hipStream_t s) {
+  if (prec == PREC_F64) return launch_prec<double>(kind, mu, (double*)lat_base, a, nblocks, block, spt, s);
+  return launch_prec<float>(kind, mu, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, s);
+}
+
+hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
+                       hipStream_t s) {
+  // single lattices only; whole wavefronts within the kernel's launch bound;
+  // the plane decomposition of a.single must be one chunk per plane
+  const int maxb = prec == PREC_F64 ? 512 : 1024;
+  if (a.reps != nullptr || block < 64 || block > maxb || block % 64 != 0 || a.single.bpp != 1 ||
+      !(spt == 1 || spt == 2 || (spt == 4 && prec != PREC_F64)) || (a.meas != nullptr && a.ml != 0))
+    return hipErrorInvalidValue;
+  if (prec == PREC_F64) return launch_pair_prec<double>(kind, (double*)lat_base, a, nblocks, block, spt, s);
+  return launch_pair_prec<float>(kind, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, s);
 }
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t s) {

@@ -137,6 +137,47 @@ int main() {
       }
     }
   }
+  // fused-pair decomposition: one chunk (workgroup) per plane, the plane
+  // covered in steps of spt * block sites (tail threads in bounds)
+  long pairsites = 0;
+  for (const auto& G : geoms) {
+    u1d::RepDesc r{};
+    for (int i = 0; i < 4; ++i) r.N[i] = G[i];
+    r.nxh = G[0] / 2;
+    r.V = G[0] * G[1] * G[2] * G[3];
+    r.HV = r.V / 2;
+    r.sy = r.nxh; r.sz = r.nxh * G[1]; r.st = r.nxh * G[1] * G[2];
+    u1d::MagicDiv a = u1d::make_magic(r.nxh), b = u1d::make_magic(G[1]), c = u1d::make_magic(G[2]);
+    r.div_nxh_m = a.m; r.div_nxh_s = a.s; r.div_ny_m = b.m; r.div_ny_s = b.s; r.div_nz_m = c.m; r.div_nz_s = c.s;
+    const uint32_t P = (uint32_t)(r.nxh * G[1]);
+    if (r.V > 1000000) continue;
+    for (int tiled = 0; tiled < 2; ++tiled)
+      for (int step : {64, 128, 576, 1024, 2048}) {
+        u1d::set_phase_blocks(r, (int)P);
+        u1d::set_phase_tiles(r, tiled != 0);
+        if (tiled && !r.tile) continue;
+        if (r.bpp != 1 || r.nblk != G[2] * G[3]) { printf("pair: not one chunk per plane\n"); return 1; }
+        std::vector<unsigned char> seen(r.HV, 0);
+        for (int blk = 0; blk < r.nblk; ++blk) {
+          uint32_t t0 = 0;
+          for (uint32_t base = 0; base < P; base += (uint32_t)step)
+            for (uint32_t tid = 0; tid < (uint32_t)step; ++tid) {
+              u1d::PhaseSite ps;
+              u1d::phase_site(r, blk, base + tid, (uint32_t)step, 0, ps);
+              if (ps.id >= (uint32_t)r.HV) { printf("pair: id out of range\n"); return 1; }
+              if (base == 0 && tid == 0) t0 = ps.t;
+              if (ps.t != t0) { printf("pair: t not uniform over the workgroup\n"); return 1; }
+              if (!ps.valid) continue;
+              if (seen[ps.id]) { printf("pair: duplicate site\n"); return 1; }
+              seen[ps.id] = 1;
+              ++pairsites;
+            }
+        }
+        for (int k = 0; k < r.HV; ++k)
+          if (!seen[k]) { printf("pair: site %d not covered\n", k); return 1; }
+      }
+  }
+  printf("pair decomposition ok (%ld site visits)\n", pairsites);
   printf("phase decomposition ok (%ld site visits)\n", psites);
   printf("indexing ok (%ld site/direction pairs)\n", checked);
   return 0;

@@ -57,10 +57,12 @@ def test_tune_cache_roundtrip(tmp_path):
     p = tmp_path / "tunecache.csv"
     p.write_text("tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n"
                  "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t0\t1\t1\t0\t# x\n"
-                 "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t0\t1\t1\t0\t# x\n")
+                 "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t0\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tOverRelaxationPair_f32\tthreads=halfvolume\t576\t2\t2\t0\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t0\t1\t1\t0\t# x\n")
     u1.tune_load(str(p))
     n = C.c_int32(0)
-    assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 2
+    assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 4
     q = tmp_path / "out.csv"
     u1.tune_save(str(q))
     txt = q.read_text()
@@ -68,6 +70,9 @@ def test_tune_cache_roundtrip(tmp_path):
     # block.z = 1 + XCD order + 2 x tiled planes survives the round trip
     assert "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t" in txt
     assert "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t" in txt
+    # fused pair shapes (workgroup = whole plane, up to 1024 threads) and "two launches" (block 0)
+    assert "8x8x8x8\tOverRelaxationPair_f32\tthreads=halfvolume\t576\t2\t2\t" in txt
+    assert "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t" in txt
     bad = tmp_path / "bad.csv"
     bad.write_text("not a cache\n")
     with pytest.raises(u1.U1Error):
@@ -78,13 +83,18 @@ def test_tune_cache_roundtrip(tmp_path):
         u1.tune_load(str(stale))
 
 
-@pytest.mark.parametrize("shape", ["100\t1\t1", "0\t1\t1", "1024\t1\t1", "128\t3\t1", "128\t1\t5", "128\t1\t0",
-                                   "64\t4\t2"])
-def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, shape):
+@pytest.mark.parametrize("name,shape", [("Metropolis_f64", s) for s in ("100\t1\t1", "0\t1\t1", "1024\t1\t1",
+                                                                       "128\t3\t1", "128\t1\t5", "128\t1\t0",
+                                                                       "64\t4\t2")]
+                         + [("OverRelaxationPair_f32", "1088\t2\t1"), ("OverRelaxationPair_f32", "600\t2\t1"),
+                            ("MetropolisPair_f64", "1024\t1\t1"), ("MetropolisPair_f64", "256\t4\t1"),
+                            ("MetropolisPair_f64", "0\t2\t1")])
+def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, name, shape):
     """A cache entry must be one of the tuner's own shapes (whole wavefronts,
     <= 512 threads, 1/2/4 sites per thread, 4 only for binary32; block.z 1-4 =
-    1 + XCD order + 2 x tiled planes), and a file with one bad line changes
-    nothing."""
+    1 + XCD order + 2 x tiled planes; fused pairs: whole wavefronts up to 1024
+    threads in binary32 and 512 in binary64, or block 0 with one site per
+    thread for two launches), and a file with one bad line changes nothing."""
     import latticeu1_amd as u1
 
     lib = u1.load()
@@ -93,7 +103,7 @@ def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, shape):
     n0 = C.c_int32(0)
     assert lib.u1_tune_entries(C.byref(n0)) == 0
     p = tmp_path / "bad_shape.csv"
-    p.write_text(head + good + f"6x6x6x6\tMetropolis_f64\tthreads=halfvolume\t{shape}\t0\t1\t1\t0\t# x\n")
+    p.write_text(head + good + f"6x6x6x6\t{name}\tthreads=halfvolume\t{shape}\t0\t1\t1\t0\t# x\n")
     with pytest.raises(u1.U1Error) as e:
         u1.tune_load(str(p))
     assert e.value.code == 5 and "launch shape" in str(e.value)

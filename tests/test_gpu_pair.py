@@ -1,0 +1,109 @@
+"""Fused (parity, 0) + (parity, 1) update launches (k_pair, u1_set_pair_fusion).
+
+One workgroup owns a whole (z, t) plane and runs both phases with a workgroup
+barrier between them.  The bar is the unfused path's: FP64 bit-identical to
+the oracle after every sweep with identical accept counts; FP32 bit-identical
+to the FP32 two-launch path.  Fusion is forced on (mode 2) here, with the
+tuner on and off, on planes that fit one step, need several steps per phase
+(P > workgroup x sites per thread), or leave most of the workgroup idle.
+"""
+import numpy as np
+import pytest
+
+from tests.test_gpu_parity import bits, make_pair, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def forced(u1):
+    u1.set_pair_fusion(2)
+    yield
+    u1.set_pair_fusion(1)
+    u1.tune_enable(True)
+
+
+@pytest.mark.parametrize("dims,beta,aniso,metrop,ovrn,tune", [
+    ((8, 8, 8, 8), 1.0, 1.0, 1, 3, False),    # P = 32: one mostly idle workgroup per plane
+    ((6, 10, 4, 12), 0.9, 1.0, 2, 1, True),    # ragged extents, tuned pair shape
+    ((4, 4, 4, 8), 1.5, 2.0, 1, 3, False),     # anisotropic
+    ((48, 24, 4, 4), 1.0, 1.0, 1, 2, False),   # P = 576 > 512: two steps per phase in binary64
+    ((2, 2, 2, 2), 1.0, 1.0, 1, 3, False),     # every neighbour wraps
+])
+def test_fp64_pair_parity(u1, orc, forced, dims, beta, aniso, metrop, ovrn, tune):
+    u1.tune_enable(tune)
+    g, og, lat, ref = make_pair(u1, orc, dims, beta, aniso, True, seed=41)
+    say(f"fp64 pair parity {dims} beta={beta} xi={aniso} metrop={metrop} ovrn={ovrn} tune={tune}")
+    m = u1.BENCH_METRO
+    om = orc.metro(1, 4, 0.5)
+    for s in range(10):
+        assert lat.update(metrop, ovrn, m, 41, s, count_accepts=True) == \
+            orc.update(og, ref, metrop, ovrn, om, 41, s, count_accepts=True), s
+        assert np.array_equal(bits(lat.to_host()), bits(ref)), s
+    # the measuring phase (1, 1) runs inside the last pass's second pair
+    p = lat.update_plaquette(metrop, ovrn, m, 41, 10)
+    orc.update(og, ref, metrop, ovrn, om, 41, 10)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    assert np.max(np.abs(p - orc.plaquette(og, ref))) <= 1e-13
+
+
+def test_fp64_pair_metropolis_only_measurement(u1, orc, forced):
+    """Metropolis-only sweeps: the last Metropolis pass measures, through its pair."""
+    u1.tune_enable(False)
+    g, og, lat, ref = make_pair(u1, orc, (6, 10, 4, 12), 1.0, 1.0, True, seed=8)
+    for s in range(3):
+        p = lat.update_plaquette(2, 0, u1.BENCH_METRO, 8, s)
+        orc.update(og, ref, 2, 0, orc.metro(1, 4, 0.5), 8, s)
+        assert np.array_equal(bits(lat.to_host()), bits(ref))
+        assert np.max(np.abs(p - orc.plaquette(og, ref))) <= 1e-13
+
+
+def test_fp64_pair_masked_and_multilevel(u1, orc, forced):
+    """Multilevel masks (t % ml == 0 spatial links frozen, uniform over a
+    plane) through the pair kernel, eagerly and inside the captured graph."""
+    u1.tune_enable(False)
+    dims = (4, 4, 4, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, seed=9)
+    for ml, stream in ((4, 3), (2, 4)):
+        for s in range(3):
+            lat.update(1, 3, u1.BENCH_METRO, 9, s, stream=stream, c3=5, mask_ml=ml)
+            orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 9, s, stream=stream, c3=5, mask_ml=ml)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    mh = u1.metro(1, 20, 0.5)
+    r_gpu = lat.multilevel(2, 1, 3, 1, 1, 2, u1.BENCH_METRO, 19, call_id=2, mh_mode=1, mh=mh)
+    r_orc = orc.multilevel(og, ref, 2, 1, 3, 1, 1, 2, orc.metro(1, 4, 0.5), 19, call_id=2, mh_mode=1,
+                           mh=orc.metro(1, 20, 0.5))
+    assert np.allclose(r_gpu, r_orc, rtol=1e-11, atol=1e-16)
+
+
+@pytest.mark.parametrize("dims,tune", [((16, 16, 16, 16), False), ((48, 48, 4, 4), False),
+                                       ((48, 48, 4, 4), True), ((24, 24, 8, 8), True)])
+def test_fp32_pair_matches_two_launches(u1, dims, tune):
+    """FP32 has no oracle: the fused path is pinned to the two-launch FP32
+    path, bit for bit, over sweeps with and without the fused plaquette."""
+    g = u1.geom(dims, 1.0)
+    out = {}
+    try:
+        u1.tune_enable(tune)
+        for mode in (0, 2):
+            u1.set_pair_fusion(mode)
+            lat = u1.Lattice.cold(g, u1.F32)
+            lat.hotstart(13)
+            acc = [lat.update(1, 3, u1.BENCH_METRO, 13, s, count_accepts=True) for s in range(4)]
+            p = lat.update_plaquette(1, 3, u1.BENCH_METRO, 13, 4)
+            out[mode] = (lat.to_host(), acc, p)
+    finally:
+        u1.set_pair_fusion(1)
+        u1.tune_enable(True)
+    say(f"fp32 pair vs two launches {dims} tune={tune}: accepts {out[2][1]}")
+    assert np.array_equal(out[0][0], out[2][0])
+    assert out[0][1] == out[2][1]
+    assert np.max(np.abs(out[0][2] - out[2][2])) <= 1e-12
+
+
+def test_pair_fusion_mode_checked(u1):
+    with pytest.raises(u1.U1Error):
+        u1.set_pair_fusion(3)
+    with pytest.raises(u1.U1Error):
+        u1.set_pair_fusion(-1)
+    u1.set_pair_fusion(1)
