@@ -15,9 +15,14 @@ so a sweep is 16 V = 84,934,656 link updates.
 
 The JSON line carries:
   value          FP32-bulk link updates/s (whole timed region, inputs resident)
-  roofline       dominant FP32 phase kernel: algorithmic bytes per launch
-                 (9 * 4 B per link update * V/2 links) / mean launch duration
-                 from HIP events on the library stream, against 8 TB/s HBM;
+  roofline       dominant FP32 overrelaxation kernel (the (parity, mu) phase,
+                 or the fused (parity, 0) + (parity, 1) pair where the tuner
+                 chose it, whichever takes more time per sweep): algorithmic
+                 bytes per launch (phase: 4V angles read + V/2 written; pair:
+                 4V read + V written; 4 B each) / mean launch duration from
+                 HIP events on the library stream around 32 back-to-back
+                 launches of that class in the sweep's shapes (on a scratch
+                 copy of the lattice after the timed region), against 8 TB/s;
                  traffic (HBM bytes per launch) from rocprofv3 FETCH_SIZE /
                  WRITE_SIZE counter files when given with --traffic, else null;
                  stream_reference_GBs: plain streaming kernels moving the same
@@ -43,6 +48,12 @@ from latticeu1_amd.analysis import gamma_method, jackknife  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
+# launch classes of u1_bench_updates' kernel timing
+CLASSES = ("metropolis", "overrelaxation", "overrelaxation_with_plaquette", "metropolis_pair",
+           "overrelaxation_pair", "overrelaxation_pair_with_plaquette")
+PAIR = (False, False, False, True, True, True)
+
+
 def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
     g = u1.geom(dims, beta, 1.0)
     lat = u1.Lattice.cold(g, prec)
@@ -50,21 +61,31 @@ def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
     m = u1.BENCH_METRO
     if warmup > 0:
         lat.bench(1, 3, m, seed, 0, warmup, 0)  # thermalisation (also runs the autotuner)
-    ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, plaq_mode)
+    ms, kernel_ms, series = lat.bench(1, 3, m, seed, warmup, steps, plaq_mode, kernel_timing=True)
     return lat, ms, kernel_ms, series
 
 
-def counter_traffic(fetch_csv, write_csv, kind):
-    """HBM bytes per launch of the FP32 phase kernel of `kind` (0 Metropolis,
-    1 overrelaxation; the plain variant, not the measuring one) from two
-    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each
-    scaled by what the same counter reads for the plain copy kernel
-    k_convert_precision<float, float> (Grid_Size elements of 4 B read and 4 B
-    written) in the same run.  Returns (traffic bytes, fetch factor, write factor)."""
+def class_table(kms, V, s):
+    """Per launch class: mean ms, launches per sweep, algorithmic bytes per launch."""
+    out = {}
+    for c, name in enumerate(CLASSES):
+        if kms[6 + c] > 0:
+            out[name] = {"mean_launch_ms": round(float(kms[c]), 5), "per_sweep": round(float(kms[6 + c]), 3),
+                         "bytes_per_launch": (5.0 if PAIR[c] else 4.5) * V * s}
+    return out
+
+
+def counter_traffic(fetch_csv, write_csv, pair):
+    """HBM bytes per launch of the FP32 overrelaxation kernel (the plain phase
+    k_phase, or the plain fused pair k_pair) from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for the plain copy kernel k_convert_precision<float, float>
+    (Grid_Size elements of 4 B read and 4 B written) in the same run.
+    Returns (traffic bytes, fetch factor, write factor)."""
     import csv
     import re
 
-    pat = re.compile(r"k_phase<float, \d, %d, false, \d, false>" % kind)
+    pat = re.compile(r"k_pair<float, 1, \d, false>" if pair else r"k_phase<float, \d, 1, false, \d, false>")
 
     def scan(path):
         vals, calib = [], []
@@ -165,30 +186,40 @@ def main():
         _, gerr, tau, _ = gamma_method(pm)
         err = max(err, gerr)  # binned jackknife, or the Gamma-method error if larger
         s = 4 if prec == u1.F32 else 8
-        bytes_per_launch = 4.5 * V * s  # 4V angles read + V/2 written per (parity, mu) phase
-        res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, plaq=mean,
-                         plaq_err=err, tau=tau, bytes=bytes_per_launch, elem=s)
+        cls = class_table(kms, V, s)
+        res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, cls=cls, plaq=mean,
+                         plaq_err=err, tau=tau, elem=s)
         print(f"[bench] {name}: {res[name]['rate']:.4e} link updates/s, {ms / args.steps:.3f} ms/sweep, "
-              f"<P> = {mean:.8f} +- {err:.1e}, mean phase kernel ms M/OR/OR+plaquette = "
-              f"{kms[0]:.4f}/{kms[1]:.4f}/{kms[2]:.4f}", file=sys.stderr, flush=True)
+              f"<P> = {mean:.8f} +- {err:.1e}; launches (ms x per sweep): "
+              + ", ".join(f"{k} {v['mean_launch_ms']:.4f} x {v['per_sweep']:g}" for k, v in cls.items())
+              , file=sys.stderr, flush=True)
 
     f32 = res["fp32"]
-    # dominant kernel class: the one with the larger total time per sweep; with
-    # the fused plaquette, 3 of the 24 OR launches are the measuring variant
-    n_meas = 3 if args.plaq == "fused" else 0
-    t_m = 8 * f32["kms"][0]
-    t_o = (24 - n_meas) * f32["kms"][1] + n_meas * f32["kms"][2]
-    dom = "overrelaxation" if t_o >= t_m else "metropolis"
-    dom_ms = f32["kms"][1] if dom == "overrelaxation" else f32["kms"][0]
-    achieved = f32["bytes"] / (dom_ms * 1e-3) / 1e9
+    cls = f32["cls"]
+    # dominant memory-bound kernel: the plain overrelaxation phase or the plain
+    # fused overrelaxation pair, whichever takes more time per sweep
+    def total(k):
+        return cls[k]["mean_launch_ms"] * cls[k]["per_sweep"] if k in cls else 0.0
+    dom = "overrelaxation_pair" if total("overrelaxation_pair") > total("overrelaxation") else "overrelaxation"
+    d = cls[dom]
+    achieved = d["bytes_per_launch"] / (d["mean_launch_ms"] * 1e-3) / 1e9
+    for k, v in cls.items():
+        v["achieved_GBs"] = round(v["bytes_per_launch"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
+        if PAIR[CLASSES.index(k)]:  # two phases' worth of per-phase algorithmic bytes
+            v["phase_equivalent_GBs"] = round(9.0 * V * f32["elem"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
+    sweep_bytes = sum(v["bytes_per_launch"] * v["per_sweep"] for v in cls.values())
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_phase<float> {dom} (parity, mu) phase",
-                "bytes_per_launch": f32["bytes"], "mean_launch_ms": round(dom_ms, 5),
-                "other_kernel": {"metropolis_ms": round(f32["kms"][0], 5),
-                                 "overrelaxation_ms": round(f32["kms"][1], 5),
-                                 "overrelaxation_with_plaquette_ms": round(f32["kms"][2], 5)},
-                "sweep_algorithmic_GBs": round(32 * f32["bytes"] / ((t_m + t_o) * 1e-3) / 1e9, 1)}
+                "kernel": ("k_pair<float> fused overrelaxation (parity, 0) + (parity, 1)" if dom.endswith("pair")
+                           else "k_phase<float> overrelaxation (parity, mu) phase"),
+                "bytes_per_launch": d["bytes_per_launch"], "mean_launch_ms": d["mean_launch_ms"],
+                "launch_classes": cls,
+                "kernel_timing": {"back_to_back_launches_per_class": int(f32["kms"][13]),
+                                  "ms_per_sweep_sum_of_classes": round(sum(v["mean_launch_ms"] * v["per_sweep"]
+                                                                           for v in cls.values()), 4),
+                                  "ms_per_sweep_timed_region": round(f32["ms"] / args.steps, 4)},
+                "sweep_algorithmic_bytes": sweep_bytes,
+                "sweep_algorithmic_GBs": round(sweep_bytes / (f32["ms"] / args.steps * 1e-3) / 1e9, 1)}
 
     # streaming references moving the same 36 B x V/2 bytes as one FP32 phase
     # with an ideal pattern: what this HBM delivers in practice
@@ -198,9 +229,9 @@ def main():
     roofline["frac_of_read8_write1_stream"] = round(achieved / ref["read8_write1"], 4)
 
     if args.traffic:
-        tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], 1 if dom == "overrelaxation" else 0)
+        tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], dom.endswith("pair"))
         roofline["traffic"] = round(tb)
-        roofline["traffic_over_algorithmic"] = round(tb / f32["bytes"], 3)
+        roofline["traffic_over_algorithmic"] = round(tb / d["bytes_per_launch"], 3)
         roofline["traffic_calibration"] = {"fetch_factor": round(ff, 4), "write_factor": round(fw, 4)}
 
     extra = {}
@@ -208,8 +239,7 @@ def main():
         f64 = res["fp64"]
         extra["fp64_link_updates_per_s"] = f64["rate"]
         extra["fp32_over_fp64"] = f32["rate"] / f64["rate"]
-        extra["fp64_kernel_ms"] = {"metropolis": round(f64["kms"][0], 5), "overrelaxation": round(f64["kms"][1], 5),
-                                   "overrelaxation_with_plaquette": round(f64["kms"][2], 5)}
+        extra["fp64_launch_classes"] = f64["cls"]
         d = f32["plaq"] - f64["plaq"]
         comb = math.hypot(f32["plaq_err"], f64["plaq_err"])
         extra["plaquette"] = {"fp32": [f32["plaq"], f32["plaq_err"]], "fp64": [f64["plaq"], f64["plaq_err"]],

@@ -218,11 +218,16 @@ int u1_tune_entries(int32_t* n);
  * measurement into plaq_series[4*i] after every sweep if measure_plaq: 1 fused
  * into the sweep as u1_update_plaquette, 2 a separate u1_plaquette pass) and
  * return the elapsed device time in ms, from HIP events on the library stream.
- * kernel_ms (nullable, 3 doubles): mean launch duration of the Metropolis
- * phases (kernel_ms[0]), the overrelaxation phases without measurement
- * (kernel_ms[1]) and, with measure_plaq 1 and ovrn > 0, the three measuring
- * overrelaxation phases (kernel_ms[2], else 0), from events that bracket each
- * sweep's back-to-back launch groups inside the same timed region. */
+ * The timed region holds no other events.
+ * kernel_ms (nullable, 14 doubles): kernel timing after the timed region, on
+ * a scratch copy of the lattice (the chain is not advanced): per launch class
+ * c of the sweep -- 0 Metropolis phase, 1 overrelaxation phase, 2 measuring
+ * overrelaxation phase, 3 fused Metropolis pair, 4 fused overrelaxation pair,
+ * 5 measuring fused overrelaxation pair (u1_set_pair_fusion) -- 32
+ * back-to-back launches in the sweep's shapes between one event pair;
+ * kernel_ms[c] is the mean launch duration (0 if the sweep has none) and
+ * kernel_ms[6 + c] the launches per sweep; kernel_ms[12] is the timed
+ * region's ms per sweep and kernel_ms[13] the launches timed per class. */
 int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms);

@@ -352,14 +352,19 @@ class Lattice:
         h = F.to_host()
         return (h[0::2] + 1j * h[1::2]).reshape(6, size), means[0::2] + 1j * means[1::2]
 
-    def bench(self, metrop: int, ovrn: int, m: Metro, seed: int, sweep0: int, nsweeps: int, measure_plaq: int):
-        """measure_plaq: 0 none, 1 fused into the sweep, 2 separate plaquette pass."""
+    def bench(self, metrop: int, ovrn: int, m: Metro, seed: int, sweep0: int, nsweeps: int, measure_plaq: int,
+              kernel_timing: bool = False):
+        """measure_plaq: 0 none, 1 fused into the sweep, 2 separate plaquette pass.
+        kernel_timing: afterwards, time every launch class of the sweep on a
+        scratch copy (the chain is not advanced); returns the 14 values of
+        u1_bench_updates' kernel_ms (else None)."""
         measure_plaq = int(measure_plaq)
         series = np.zeros(4 * nsweeps) if measure_plaq else None
         el = C.c_double(0)
-        km = np.zeros(3)
+        km = np.zeros(14) if kernel_timing else None
         call("u1_bench_updates", C.byref(self.g), self.prec, self.buf.ptr, metrop, ovrn, C.byref(m), seed, sweep0,
-             nsweeps, measure_plaq, _dp(series) if measure_plaq else None, C.byref(el), _dp(km))
+             nsweeps, measure_plaq, _dp(series) if measure_plaq else None, C.byref(el),
+             _dp(km) if kernel_timing else None)
         return el.value, km, (series.reshape(nsweeps, 4) if measure_plaq else None)
 
 

@@ -329,17 +329,23 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
 }
 
 // Fused pair shapes of a geometry: whole (z, t) planes per workgroup, SPT
-// sites per thread per step, the workgroup size rounded up to whole
-// wavefronts and capped at the kernel's bound (larger planes take more steps).
+// sites per thread per step, the workgroup size for one step over the plane
+// and for two, rounded up to whole wavefronts and capped at the kernel's
+// bound (larger planes take more steps).
 std::vector<Cfg> pair_candidates(int prec, const RepDesc& r) {
   const int P = r.nxh * r.N[1];
   const int maxb = prec == U1_F64 ? 512 : 1024;
   std::vector<Cfg> v;
   for (int spt : {1, 2, 4}) {
     if (spt == 4 && prec != U1_F32) continue;
-    const int b = std::min(maxb, ((P + spt - 1) / spt + 63) / 64 * 64);
-    for (int t = 0; t < 2; ++t)
-      for (int x = 0; x < 2; ++x) v.push_back(Cfg{b, spt, x, t});
+    int prev = 0;
+    for (int steps = 1; steps <= 2; ++steps) {
+      const int b = std::min(maxb, ((P + spt * steps - 1) / (spt * steps) + 63) / 64 * 64);
+      if (b == prev) continue;
+      prev = b;
+      for (int t = 0; t < 2; ++t)
+        for (int x = 0; x < 2; ++x) v.push_back(Cfg{b, spt, x, t});
+    }
   }
   return v;
 }
@@ -467,14 +473,6 @@ int settle_update_cfgs(int prec, const u1_geom* g, const void* lat, const PhaseA
   return U1_OK;
 }
 
-// Events around one sweep's launch groups: start | Metropolis | after_metro |
-// OR phases without measurement | before_meas | measuring phases (and the OR
-// phases between them) | after_or.  Without measurement before_meas is
-// recorded together with after_or.
-struct GroupEvents {
-  hipEvent_t start, after_metro, before_meas, after_or;
-};
-
 // One UpdateLattice on a single lattice (update.cu:449-456 loop order).
 // meas_dst (device, 4 doubles, nullable): the plaquette of the updated
 // lattice, in u1_plaquette's normalisation, from the measuring variants of
@@ -483,7 +481,7 @@ struct GroupEvents {
 // sweep, read by the kernels (for graph capture; phase_cfg must have run).
 int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, const u1_metro* m, uint64_t seed,
                uint32_t stream, uint32_t sweep, uint32_t c3, int ml, unsigned long long* acc_dev,
-               const GroupEvents* gev = nullptr, double* meas_dst = nullptr, const uint32_t* sweep_dev = nullptr,
+               double* meas_dst = nullptr, const uint32_t* sweep_dev = nullptr,
                uint32_t sweep_mul = 0) {
   // the Metropolis pass index takes 12 bits of Philox counter word 2
   if (metrop < 0 || ovrn < 0 || metrop > 4095) return fail(U1_ERR_ARG, "bad pass counts");
@@ -525,38 +523,29 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   // one pass of kind `kind`: per parity, (parity, 0) + (parity, 1) fused or
   // not, then (parity, 2) and (parity, 3)
   auto pass = [&](int kind, uint32_t p, bool last, const Cfg& c, const RepDesc& r, const Cfg& pc,
-                  const RepDesc& rp, bool* marked) -> int {
+                  const RepDesc& rp) -> int {
     a.pass = p;
     for (int parity = 0; parity < 2; ++parity)
       for (int mu = 0; mu < 4; ++mu) {
+        const bool pair = mu == 0 && pc.block != 0;
         a.parity = parity;
-        a.meas = meas_rows(last, parity, mu == 0 && pc.block != 0 ? 1 : mu);
-        if (gev && marked && a.meas && !*marked) {
-          HIPCHK(hipEventRecord(gev->before_meas, G.st));
-          *marked = true;
-        }
-        if (mu == 0 && pc.block != 0) {
+        a.meas = meas_rows(last, parity, pair ? 1 : mu);
+        if (pair) {
           a.single = rp;
           a.xcd = (uint32_t)pc.xcd;
           HIPCHK(u1d::launch_pair(prec, kind, lat, a, rp.nblk, pc.block, pc.spt, G.st));
           ++mu;
-          continue;
+        } else {
+          a.single = r;
+          a.xcd = (uint32_t)c.xcd;
+          HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, r.nblk, c.block, c.spt, G.st));
         }
-        a.single = r;
-        a.xcd = (uint32_t)c.xcd;
-        HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, r.nblk, c.block, c.spt, G.st));
       }
     return U1_OK;
   };
-  if (gev) HIPCHK(hipEventRecord(gev->start, G.st));
   for (int p = 0; p < metrop; ++p)
-    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, ovrn == 0 && p == metrop - 1, cm, rm, pm, rpm, nullptr));
-  if (gev) HIPCHK(hipEventRecord(gev->after_metro, G.st));
-  bool marked = false;
-  for (int o = 0; o < ovrn; ++o)
-    RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == ovrn - 1, co, ro, po, rpo, &marked));
-  if (gev && !marked) HIPCHK(hipEventRecord(gev->before_meas, G.st));
-  if (gev) HIPCHK(hipEventRecord(gev->after_or, G.st));
+    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, ovrn == 0 && p == metrop - 1, cm, rm, pm, rpm));
+  for (int o = 0; o < ovrn; ++o) RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == ovrn - 1, co, ro, po, rpo));
   if (meas_dst)
     HIPCHK(u1d::launch_finish_sums(G.partials, meas_rows_total, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
                                    fold_scratch_ptr(meas_rows_total, 4), G.st));
@@ -904,7 +893,7 @@ int u1_update_plaquette(const u1_geom* g, int prec, void* lat, int32_t metrop, i
   RET(check_metro(m));
   if (!lat || !out) return fail(U1_ERR_ARG, "null argument");
   if (metrop < 0 || ovrn < 0 || metrop > 4095 || metrop + ovrn == 0) return fail(U1_ERR_ARG, "bad pass counts");
-  RET(run_update(g, prec, lat, metrop, ovrn, m, seed, stream, sweep, c3, 0, nullptr, nullptr, G.dres));
+  RET(run_update(g, prec, lat, metrop, ovrn, m, seed, stream, sweep, c3, 0, nullptr, G.dres));
   return copy_back(out, G.dres, 4);
 }
 
@@ -1096,7 +1085,7 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   MLCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
   for (int l = 0; l < k2 && rc == U1_OK; ++l)
     rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)l, call_id, 2, nullptr, nullptr,
-                    nullptr, ctr, (uint32_t)k2);
+                    ctr, (uint32_t)k2);
   if (rc == U1_OK) {
     const hipError_t e1 = u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT, 0,
                                                      call_id, (double*)tl, 256, G.st, ctr);
@@ -1548,6 +1537,93 @@ int u1_tune_entries(int32_t* n) {
 }
 
 // ---------------------------------------------------------------- timing helper
+// Launch classes of a sweep, for kernel timing.
+enum LaunchClass { LC_METRO = 0, LC_OR, LC_OR_MEAS, LC_METRO_PAIR, LC_OR_PAIR, LC_OR_PAIR_MEAS, LC_COUNT };
+constexpr int kTimingLaunches = 32;  // back-to-back launches per class
+
+// Kernel timing of the launch classes of a sweep (1 <= metrop + ovrn, shapes
+// already settled): on a scratch copy of the lattice, kTimingLaunches
+// back-to-back launches of each class in the sweep's shapes (parity and mu
+// cycling over the phases the class covers in a pass), bracketed by one event
+// pair.  ms[c]: mean launch duration; per_sweep[c]: launches per sweep.
+int time_launch_classes(const u1_geom* g, int prec, const void* lat, int metrop, int ovrn, const u1_metro* m,
+                        uint64_t seed, uint32_t sweep, bool meas, double* ms, double* per_sweep) {
+  PhaseArgs a{};
+  a.single = make_rep(g, seed, 0);
+  a.proposal = m->proposal;
+  a.nhit = m->nhit;
+  a.step = m->step;
+  a.stream = U1_STREAM_UPDATE;
+  a.sweep = sweep;
+  Cfg cfg[2] = {kDefaultCfg, kDefaultCfg}, pcfg[2] = {{0, 1, 0, 0}, {0, 1, 0, 0}};
+  const int passes[2] = {metrop, ovrn};
+  for (int kind = 0; kind < 2; ++kind)
+    if (passes[kind] > 0) {
+      RET(phase_cfg(prec, kind, g, lat, a, &cfg[kind]));
+      RET(pair_cfg(prec, kind, g, lat, a, cfg[kind], &pcfg[kind]));
+    }
+  const int64_t bytes = 4 * (int64_t)a.single.V * (int64_t)elem_size(prec);
+  DevBuf scratch;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(hipMemcpyAsync(scratch.p, lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  EventSet es;
+  hipEvent_t e0, e1;
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
+  for (int c = 0; c < LC_COUNT; ++c) ms[c] = per_sweep[c] = 0.0;
+  // with the plaquette, the last overrelaxation pass measures in its parity-1
+  // launches of mu >= 1 (the parity-1 pair, when fused)
+  const bool meas_or = meas && ovrn > 0;
+  for (int c = 0; c < LC_COUNT; ++c) {
+    const bool pair = c >= LC_METRO_PAIR;
+    const int kind = (c == LC_METRO || c == LC_METRO_PAIR) ? u1d::KIND_METROPOLIS : u1d::KIND_OVERRELAX;
+    const bool cmeas = c == LC_OR_MEAS || c == LC_OR_PAIR_MEAS;
+    const Cfg& pc = pcfg[kind];
+    const bool fused = pc.block != 0;
+    if (passes[kind] == 0 || (pair && !fused) || (cmeas && !meas_or)) continue;
+    // the (parity, mu) launches of one pass in this class (a pair is listed by mu 0)
+    std::vector<std::pair<int, int>> ph;
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        if (pair ? mu != 0 : (fused && mu < 2)) continue;
+        const bool is_meas = parity == 1 && (pair || mu >= 1);
+        if (!cmeas || is_meas) ph.push_back({parity, mu});
+      }
+    if (cmeas)
+      per_sweep[c] = (double)ph.size();
+    else {
+      const int n_meas = meas_or && kind == u1d::KIND_OVERRELAX ? (pair ? 1 : (fused ? 2 : 3)) : 0;
+      per_sweep[c] = (double)ph.size() * passes[kind] - n_meas;
+    }
+    PhaseArgs b = a;
+    b.xcd = (uint32_t)(pair ? pc.xcd : cfg[kind].xcd);
+    if (pair)
+      set_pair_blocks(b.single, pc);
+    else
+      set_update_blocks(b.single, cfg[kind]);
+    if (cmeas) RET(ensure_fold(b.single.nblk, 4));
+    auto launch = [&](int i) -> int {
+      b.parity = ph[(size_t)i % ph.size()].first;
+      b.pass = (uint32_t)(i / (int)ph.size());
+      b.meas = cmeas ? G.partials : nullptr;
+      if (pair) HIPCHK(u1d::launch_pair(prec, kind, scratch.p, b, b.single.nblk, pc.block, pc.spt, G.st));
+      else
+        HIPCHK(u1d::launch_phase(prec, kind, ph[(size_t)i % ph.size()].second, scratch.p, b, b.single.nblk,
+                                 cfg[kind].block, cfg[kind].spt, G.st));
+      return U1_OK;
+    };
+    RET(launch(0));  // warm-up
+    HIPCHK(hipEventRecord(e0, G.st));
+    for (int i = 0; i < kTimingLaunches; ++i) RET(launch(i));
+    HIPCHK(hipEventRecord(e1, G.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    ms[c] = t / kTimingLaunches;
+  }
+  return U1_OK;
+}
+
 int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms) {
@@ -1571,16 +1647,8 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
     a.step = m->step;
     RET(settle_update_cfgs(prec, g, lat, a, metrop, ovrn));
   }
-  // Timed region: every sweep brackets its Metropolis and OR launch groups
-  // with events (kernel_ms), plus one event pair around the whole region.
+  // Timed region: one event pair around the whole region, nothing inside.
   EventSet es;
-  std::vector<GroupEvents> gev(kernel_ms ? (size_t)nsweeps : 0);
-  for (auto& e : gev) {
-    HIPCHK(es.create(&e.start));
-    HIPCHK(es.create(&e.after_metro));
-    HIPCHK(es.create(&e.before_meas));
-    HIPCHK(es.create(&e.after_or));
-  }
   hipEvent_t e0, e1;
   HIPCHK(es.create(&e0));
   HIPCHK(es.create(&e1));
@@ -1588,7 +1656,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   HIPCHK(hipEventRecord(e0, G.st));
   for (int s = 0; s < nsweeps; ++s) {
     RET(run_update(g, prec, lat, metrop, ovrn, m, seed, U1_STREAM_UPDATE, sweep0 + (uint32_t)s, 0, 0, nullptr,
-                   kernel_ms ? &gev[s] : nullptr, measure_plaq == 1 ? series + 4 * s : nullptr));
+                   measure_plaq == 1 ? series + 4 * s : nullptr));
     if (measure_plaq == 2) RET(plaquette_async(g, prec, lat, series + 4 * s));
   }
   HIPCHK(hipEventRecord(e1, G.st));
@@ -1597,23 +1665,10 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *elapsed_ms = ms;
   if (kernel_ms) {
-    // measuring OR phases: (parity 1, mu 1..3) of the last pass, i.e. the last
-    // three OR launches of a sweep
-    const bool meas_or = measure_plaq == 1 && ovrn > 0;
-    const int n_or_plain = 8 * ovrn - (meas_or ? 3 : 0);
-    double tm = 0, to = 0, tq = 0;
-    for (auto& e : gev) {
-      float a = 0, b = 0, c = 0;
-      HIPCHK(hipEventElapsedTime(&a, e.start, e.after_metro));
-      HIPCHK(hipEventElapsedTime(&b, e.after_metro, e.before_meas));
-      HIPCHK(hipEventElapsedTime(&c, e.before_meas, e.after_or));
-      tm += a;
-      to += b;
-      tq += c;
-    }
-    kernel_ms[0] = metrop > 0 ? tm / (8.0 * metrop * nsweeps) : 0.0;
-    kernel_ms[1] = n_or_plain > 0 ? to / ((double)n_or_plain * nsweeps) : 0.0;
-    kernel_ms[2] = meas_or ? tq / (3.0 * nsweeps) : 0.0;
+    RET(time_launch_classes(g, prec, lat, metrop, ovrn, m, seed, sweep0 + (uint32_t)nsweeps, measure_plaq == 1,
+                            kernel_ms, kernel_ms + LC_COUNT));
+    kernel_ms[2 * LC_COUNT] = (double)ms / nsweeps;
+    kernel_ms[2 * LC_COUNT + 1] = kTimingLaunches;
   }
   if (measure_plaq && plaq_series)
     HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));

@@ -107,3 +107,29 @@ def test_pair_fusion_mode_checked(u1):
     with pytest.raises(u1.U1Error):
         u1.set_pair_fusion(-1)
     u1.set_pair_fusion(1)
+
+
+def test_kernel_timing_classes(u1):
+    """u1_bench_updates' kernel timing (back-to-back launches per class on a
+    scratch copy): launches per sweep by class, with and without fused pairs
+    (1 Metropolis + 3 OR passes, plaquette fused into the last pass), and the
+    same plaquette series either way."""
+    g = u1.geom((8, 8, 8, 8), 1.0)
+    try:
+        u1.tune_enable(False)
+        got = {}
+        for mode in (0, 2):
+            u1.set_pair_fusion(mode)
+            lat = u1.Lattice.cold(g, u1.F32)
+            lat.hotstart(3)
+            ms, km, series = lat.bench(1, 3, u1.BENCH_METRO, 3, 0, 12, 1, kernel_timing=True)
+            got[mode] = (ms, km, series)
+            assert km[13] == 32 and ms > 0 and abs(km[12] - ms / 12) <= 1e-6 * ms
+            assert all(km[c] > 0 for c in range(6) if km[6 + c] > 0)
+    finally:
+        u1.set_pair_fusion(1)
+        u1.tune_enable(True)
+    # metropolis, or, or+plaq, metropolis pair, or pair, or pair+plaq (launches per sweep)
+    assert list(got[0][1][6:12]) == [8, 21, 3, 0, 0, 0]
+    assert list(got[2][1][6:12]) == [4, 10, 2, 2, 5, 1]
+    assert np.array_equal(got[0][2], got[2][2]) or np.max(np.abs(got[0][2] - got[2][2])) <= 1e-12
