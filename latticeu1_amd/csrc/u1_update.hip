@@ -1,9 +1,11 @@
 // u1_update.hip -- checkerboard Metropolis and overrelaxation phases, hot start.
 //
 // One launch per (pass, parity, mu) phase, as the reference (update.cu:177-190,
-// 227-231, 301-316, 379-394).  Within a phase no link reads a link of the same
-// phase: the staple of (x, mu) reads theta_nu (nu != mu) and theta_mu at the
-// other parity (staple.h:85-92), so the launch is race free.
+// 227-231, 301-316, 379-394), or one k_pair launch for the phases (parity, 0)
+// and (parity, 1) of a single lattice (see k_pair).  Within a phase no link
+// reads a link of the same phase: the staple of (x, mu) reads theta_nu
+// (nu != mu) and theta_mu at the other parity (staple.h:85-92), so the launch
+// is race free.
 //
 // Work decomposition (instruction-issue bound, so indexing is kept scalar):
 // a workgroup covers a contiguous chunk of one (z, t) plane of the half
