@@ -117,7 +117,7 @@ def test_bessel_against_scipy(orc):
     i0, i1 = orc.bessel_i0i1(x)
     assert np.allclose(i0, sp.i0(x), rtol=1e-14, atol=0)
     assert np.allclose(i1, sp.i1(x), rtol=1e-14, atol=0)
-    # the known values quoted in the survey (std::cyl_bessel_i probe)
+    # known values: libstdc++ std::cyl_bessel_i(0 and 1, 1.5)
     a, b = orc.bessel_i0i1(np.array([1.5]))
     assert abs(a[0] - 1.6467231897728907) < 4e-16 and abs(b[0] - 0.98166642857790731) < 4e-16
 
