@@ -619,6 +619,12 @@ struct u1_ensemble {
   hipGraphExec_t mexec = nullptr;
   Cfg cfg = {256, 1, 1, 0};  // launch shape of the update phases (tuned on first use)
   bool tuned = false;
+  // fused (parity, 0) + (parity, 1) launches (block 0: none): per-replica
+  // plane tables with one workgroup per (z, t) plane
+  Cfg pcfg = {0, 1, 0, 0};
+  RepDesc* pair_dev = nullptr;
+  uint16_t* pair_map = nullptr;
+  int pair_blocks = 0;
 };
 
 namespace {
@@ -650,8 +656,48 @@ int ensemble_build_update(u1_ensemble* e, Cfg c) {
   return U1_OK;
 }
 
+int ensemble_tune(u1_ensemble* e);
+int ensemble_build_pair(u1_ensemble* e, Cfg c);
+// Settles the ensemble's launch shapes before either graph exists (they fix
+// the launch layout): tuned, or, with tuning off and fused pairs forced
+// (u1_set_pair_fusion(2)), a default pair shape.
+int ensemble_settle(u1_ensemble* e) {
+  if (e->tuned || e->exec || e->mexec) return U1_OK;
+  if (G.tune) return ensemble_tune(e);
+  if (G.pair == 2 && e->pcfg.block == 0) return ensemble_build_pair(e, Cfg{256, 1, 1, 0});
+  return U1_OK;
+}
+
+// Per-replica tables of the fused pair launches for shape c (block 0: none).
+int ensemble_build_pair(u1_ensemble* e, Cfg c) {
+  e->pcfg = c;
+  if (c.block == 0) return U1_OK;
+  std::vector<RepDesc> reps(e->upd_host);
+  std::vector<uint16_t> pmap;
+  int nb = 0;
+  for (int i = 0; i < e->nrep; ++i) {
+    RepDesc& p = reps[i];
+    u1d::set_phase_blocks(p, p.nxh * p.N[1]);  // one chunk per plane
+    p.tile = 0;
+    p.blk_begin = nb;
+    nb += p.nblk;
+    for (int b = 0; b < p.nblk; ++b) pmap.push_back((uint16_t)i);
+  }
+  if (!e->pair_dev) HIPCHK(hipMalloc((void**)&e->pair_dev, e->nrep * sizeof(RepDesc)));
+  if (e->pair_map) HIPCHK(hipFree(e->pair_map));
+  e->pair_map = nullptr;
+  HIPCHK(hipMalloc((void**)&e->pair_map, pmap.size() * sizeof(uint16_t)));
+  HIPCHK(hipMemcpyAsync(e->pair_map, pmap.data(), pmap.size() * sizeof(uint16_t), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipMemcpyAsync(e->pair_dev, reps.data(), e->nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
+  HIPCHK(hipStreamSynchronize(G.st));
+  e->pair_blocks = nb;
+  return U1_OK;
+}
+
 // mrows (nullable): plaquette rows of the measuring phases (parity 1, mu >= 1
 // of the last pass), segment mu - 1 at row offset (mu - 1) * upd_blocks.
+// With fused pairs (e->pcfg), the phases (parity, 0) + (parity, 1) of every
+// pass run as one launch, except the measuring parity 1 of the last pass.
 int ensemble_enqueue_phases(u1_ensemble* e, void* lat, double* mrows = nullptr) {
   auto meas_rows = [&](bool last_pass, int parity, int mu) -> double* {
     if (!mrows || !last_pass || parity != 1 || mu < 1) return nullptr;
@@ -670,24 +716,30 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat, double* mrows = nullptr) 
   a.sweep_mul = 1;
   a.accepts = nullptr;
   a.xcd = (uint32_t)e->cfg.xcd;
+  const bool fused = e->pcfg.block != 0;
+  auto pass = [&](int kind, uint32_t p, bool last) -> int {
+    a.pass = p;
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        a.parity = parity;
+        if (mu == 0 && fused && !meas_rows(last, parity, 1)) {
+          PhaseArgs b = a;
+          b.reps = e->pair_dev;
+          b.blockmap = e->pair_map;
+          b.meas = nullptr;
+          b.xcd = (uint32_t)e->pcfg.xcd;
+          HIPCHK(u1d::launch_pair(e->prec, kind, lat, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt, G.st));
+          ++mu;
+          continue;
+        }
+        a.meas = meas_rows(last, parity, mu);
+        HIPCHK(u1d::launch_phase(e->prec, kind, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt, G.st));
+      }
+    return U1_OK;
+  };
   for (int p = 0; p < e->metrop; ++p)
-    for (int parity = 0; parity < 2; ++parity)
-      for (int mu = 0; mu < 4; ++mu) {
-        a.pass = (uint32_t)p;
-        a.parity = parity;
-        a.meas = meas_rows(e->ovrn == 0 && p == e->metrop - 1, parity, mu);
-        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_METROPOLIS, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
-                                 G.st));
-      }
-  for (int o = 0; o < e->ovrn; ++o)
-    for (int parity = 0; parity < 2; ++parity)
-      for (int mu = 0; mu < 4; ++mu) {
-        a.pass = (uint32_t)o;
-        a.parity = parity;
-        a.meas = meas_rows(o == e->ovrn - 1, parity, mu);
-        HIPCHK(u1d::launch_phase(e->prec, u1d::KIND_OVERRELAX, mu, lat, a, e->upd_blocks, e->cfg.block, e->cfg.spt,
-                                 G.st));
-      }
+    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, e->ovrn == 0 && p == e->metrop - 1));
+  for (int o = 0; o < e->ovrn; ++o) RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == e->ovrn - 1));
   return U1_OK;
 }
 
@@ -726,6 +778,33 @@ int ensemble_tune(u1_ensemble* e) {
     }
   }
   RET(ensemble_build_update(e, bestc));
+  // fused pairs: a few whole-plane shapes over full sweeps, against none
+  if (G.pair != 0) {
+    const Cfg none{0, 1, 0, 0};
+    float tnone = best, tp = 1e30f;
+    Cfg bestp = none;
+    for (const Cfg pc : {Cfg{64, 1, 1, 0}, Cfg{128, 1, 1, 0}, Cfg{256, 1, 1, 0}, Cfg{512, 1, 1, 0}, Cfg{128, 2, 1, 0},
+                         Cfg{256, 2, 1, 0}, Cfg{128, 1, 0, 0}, Cfg{256, 1, 0, 0}}) {
+      RET(ensemble_build_pair(e, pc));
+      RET(ensemble_enqueue_phases(e, scratch.p));  // warm-up
+      HIPCHK(hipEventRecord(e0, G.st));
+      RET(ensemble_enqueue_phases(e, scratch.p));
+      HIPCHK(hipEventRecord(e1, G.st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < tp) {
+        tp = ms;
+        bestp = pc;
+      }
+    }
+    const bool use = G.pair == 2 || tp < tnone;
+    if (G.verbosity >= 1)
+      std::fprintf(stderr, "tune ensemble (%d replicas): fused pairs %.1f us per sweep (block %d, %d site(s)/thread) "
+                   "vs %.1f us without: %s\n", e->nrep, 1e3 * tp, bestp.block, bestp.spt, 1e3 * tnone,
+                   use ? "fused" : "separate");
+    RET(ensemble_build_pair(e, use ? bestp : none));
+  }
   e->tuned = true;
   return U1_OK;
 }
@@ -741,6 +820,8 @@ void ensemble_release(u1_ensemble* e) {
   if (e->upd_dev) (void)hipFree(e->upd_dev);
   if (e->meas_dev) (void)hipFree(e->meas_dev);
   if (e->upd_map) (void)hipFree(e->upd_map);
+  if (e->pair_dev) (void)hipFree(e->pair_dev);
+  if (e->pair_map) (void)hipFree(e->pair_map);
   if (e->meas_map) (void)hipFree(e->meas_map);
   if (e->lat) (void)hipFree(e->lat);
   if (e->sweep_dev) (void)hipFree(e->sweep_dev);
@@ -1312,7 +1393,7 @@ int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph) {
   if (!e) return fail(U1_ERR_ARG, "null ensemble");
   if (nsweeps < 0) return fail(U1_ERR_ARG, "nsweeps must be >= 0");
   // tuning changes the launch layout, so it only runs before either graph exists
-  if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
+  RET(ensemble_settle(e));
   if (use_graph && !e->exec) {
     HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
     int rc = ensemble_enqueue_sweep(e);
@@ -1342,7 +1423,7 @@ int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series)
   if (nsweeps < 0) return fail(U1_ERR_ARG, "nsweeps must be >= 0");
   if (e->metrop + e->ovrn == 0) return fail(U1_ERR_ARG, "no measuring pass");
   if (nsweeps == 0) return U1_OK;
-  if (G.tune && !e->tuned && !e->exec && !e->mexec) RET(ensemble_tune(e));
+  RET(ensemble_settle(e));
   if (!e->mexec) {
     // (kept across a failed capture and reused by the next attempt, unless a
     // later tuning has given the update launches more workgroups)

@@ -38,10 +38,14 @@ enum { PREC_F64 = 0, PREC_F32 = 1 };
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
                         int spt, hipStream_t st);
 
-// The phases (parity, 0) and (parity, 1) of a single lattice in one launch
-// (k_pair): a.single must carry set_phase_blocks(r, Nx/2 * Ny), one workgroup
-// per (z, t) plane (nblocks = Nz * Nt); block a multiple of 64 up to 1024
-// (binary32) or 512 (binary64).  a.meas makes phase (parity, 1) measuring.
+// The phases (parity, 0) and (parity, 1) in one launch (k_pair): one
+// workgroup per (z, t) plane.  A single lattice: a.single carries
+// set_phase_blocks(r, Nx/2 * Ny) and nblocks = Nz * Nt.  Batched replicas
+// (a.reps, a.blockmap): every replica descriptor carries its own
+// set_phase_blocks(r, Nx/2 * Ny) with blk_begin its first workgroup, and the
+// block map covers nblocks; no measurement.  block a multiple of 64 up to
+// 1024 (binary32) or 512 (binary64).  a.meas makes phase (parity, 1)
+// measuring.
 hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
                        hipStream_t st);
 

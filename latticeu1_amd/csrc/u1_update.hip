@@ -412,11 +412,12 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 // When one step covers the plane (P <= SPT * blockDim.x, SPT <= 2), every
 // load of both phases except the two phase-0 results is issued before phase 0
 // computes, so phase 1 waits on the barrier and two cache-hit loads only.
-template <typename Real, int KIND, int SPT, bool MEAS>
+template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
 __global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_pair(Real* __restrict__ lat_base,
                                                                                      PhaseArgs a) {
   const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  const RepDesc& r = a.single;
+  const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
+  const uint32_t b = L - (uint32_t)r.blk_begin;  // the replica's plane (one chunk per plane)
   const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
   const uint32_t step = SPT * blockDim.x;
   double v0[2] = {0.0, 0.0}, v[2] = {0.0, 0.0};
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_
     // a frozen plane (multilevel mask, uniform over the workgroup) is frozen
     // for both phases: the whole workgroup leaves before the barrier
 #pragma unroll
-    for (int k = 0; k < SPT; ++k) phase_site(r, L, threadIdx.x + k * blockDim.x, step, a.parity, q0.ps[k]);
+    for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, step, a.parity, q0.ps[k]);
     if (a.ml != 0 && (q0.ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
 #pragma unroll
     for (int k = 0; k < SPT; ++k) q1.ps[k] = q0.ps[k];  // the same sites
@@ -439,14 +440,14 @@ __global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_
   } else {
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
-      if (phase_sites<0, SPT>(a, r, L, base + threadIdx.x, step, q.ps)) return;  // frozen plane
+      if (phase_sites<0, SPT>(a, r, b, base + threadIdx.x, step, q.ps)) return;  // frozen plane
       phase_load<Real, 0, SPT>(lat_base, a, r, q);
       phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q, v0, acc0);
     }
     __syncthreads();
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
-      phase_sites<1, SPT>(a, r, L, base + threadIdx.x, step, q.ps);
+      phase_sites<1, SPT>(a, r, b, base + threadIdx.x, step, q.ps);
       phase_load<Real, 1, SPT>(lat_base, a, r, q);
       phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
     }
@@ -518,7 +519,9 @@ PhaseArgs with_f32_constants(const PhaseArgs& a) {
 
 template <typename Real, int KIND, int SPT>
 hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
-  if (a.meas != nullptr)
+  if (a.reps != nullptr)  // batched replicas: no measuring pairs
+    k_pair<Real, KIND, SPT, false, true><<<nblocks, block, 0, s>>>(lat, a);
+  else if (a.meas != nullptr)
     k_pair<Real, KIND, SPT, true><<<nblocks, block, 0, s>>>(lat, a);
   else
     k_pair<Real, KIND, SPT, false><<<nblocks, block, 0, s>>>(lat, a);
@@ -552,8 +555,12 @@ hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, i
   // single lattices only; whole wavefronts within the kernel's launch bound;
   // the plane decomposition of a.single must be one chunk per plane
   const int maxb = prec == PREC_F64 ? 512 : 1024;
-  if (a.reps != nullptr || block < 64 || block > maxb || block % 64 != 0 || a.single.bpp != 1 ||
-      !(spt == 1 || spt == 2 || (spt == 4 && prec != PREC_F64)) || (a.meas != nullptr && a.ml != 0))
+  if (block < 64 || block > maxb || block % 64 != 0 || !(spt == 1 || spt == 2 || (spt == 4 && prec != PREC_F64)) ||
+      (a.meas != nullptr && (a.ml != 0 || a.reps != nullptr)))
+    return hipErrorInvalidValue;
+  // single lattice: one chunk per plane, one workgroup per plane (batched:
+  // the replica table and block map carry the same layout per replica)
+  if (a.reps == nullptr && (a.single.bpp != 1 || nblocks != a.single.N[2] * a.single.N[3]))
     return hipErrorInvalidValue;
   if (prec == PREC_F64) return launch_pair_prec<double>(kind, (double*)lat_base, a, nblocks, block, spt, s);
   return launch_pair_prec<float>(kind, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, s);

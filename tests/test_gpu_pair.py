@@ -133,3 +133,57 @@ def test_kernel_timing_classes(u1):
     assert list(got[0][1][6:12]) == [8, 21, 3, 0, 0, 0]
     assert list(got[2][1][6:12]) == [4, 10, 2, 2, 5, 1]
     assert np.array_equal(got[0][2], got[2][2]) or np.max(np.abs(got[0][2] - got[2][2])) <= 1e-12
+
+
+@pytest.mark.parametrize("tune,graph", [(False, True), (True, True), (False, False)])
+def test_ensemble_pairs_fp64_matches_oracle(u1, orc, tune, graph):
+    """Batched replicas with fused pairs forced (per-replica plane tables, one
+    workgroup per plane of every replica, planes from 2 to 96 sites covered in
+    one or more steps): every replica bit-identical to the oracle, and the
+    measuring sweep (its last parity-1 pair runs unfused) too."""
+    specs = [((4, 4, 4, 4), 0.9, 1.0, 1), ((6, 4, 4, 8), 1.05, 1.0, 0), ((8, 8, 8, 8), 1.1, 1.0, 1),
+             ((4, 4, 4, 8), 1.0, 2.0, 1), ((16, 12, 4, 6), 1.0, 1.0, 1), ((2, 2, 2, 2), 1.0, 1.0, 1)]
+    geoms = [u1.geom(d, b, x) for d, b, x, _ in specs]
+    seeds = [301 + i for i in range(len(specs))]
+    hot = [h for *_, h in specs]
+    try:
+        u1.tune_enable(tune)
+        u1.set_pair_fusion(2)
+        e = u1.Ensemble(geoms, seeds, hot, u1.F64, 1, 3, u1.BENCH_METRO)
+        say(f"ensemble fp64 fused pairs tune={tune} graph={graph}")
+        e.sweep(4, graph=graph)
+        series = e.sweep_plaquette(2)
+    finally:
+        u1.set_pair_fusion(1)
+        u1.tune_enable(True)
+    for r, (d, b, x, h) in enumerate(specs):
+        og = orc.geom(d, b, x)
+        ref = orc.hotstart(og, seeds[r], replica=r) if h else orc.cold(og)
+        for s in range(4):
+            orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), seeds[r], s, c3=r)
+        for s in range(4, 6):
+            orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), seeds[r], s, c3=r)
+            assert np.max(np.abs(series[s - 4, r] - orc.plaquette(og, ref))) <= 1e-13, (r, s)
+        assert np.array_equal(bits(e.read(r)), bits(ref)), f"replica {r}"
+    e.destroy()
+
+
+def test_ensemble_pairs_fp32_match_unfused(u1):
+    """FP32 batched replicas: fused pairs forced against no fusion, bit for bit."""
+    dims = [(8, 8, 8, 8), (12, 12, 12, 16), (6, 4, 4, 8), (24, 24, 8, 8)]
+    geoms = [u1.geom(d, 1.0) for d in dims]
+    seeds = [71, 72, 73, 74]
+    out = {}
+    try:
+        u1.tune_enable(False)
+        for mode in (0, 2):
+            u1.set_pair_fusion(mode)
+            e = u1.Ensemble(geoms, seeds, [1, 1, 1, 1], u1.F32, 1, 3, u1.BENCH_METRO)
+            e.sweep(5, graph=True)
+            out[mode] = [e.read(r) for r in range(len(dims))]
+            e.destroy()
+    finally:
+        u1.set_pair_fusion(1)
+        u1.tune_enable(True)
+    for r in range(len(dims)):
+        assert np.array_equal(out[0][r], out[2][r]), r
