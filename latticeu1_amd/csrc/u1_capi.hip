@@ -670,7 +670,7 @@ int ensemble_settle(u1_ensemble* e) {
 
 // Per-replica tables of the fused pair launches for shape c (block 0: none).
 int ensemble_build_pair(u1_ensemble* e, Cfg c) {
-  e->pcfg = c;
+  e->pcfg = Cfg{0, 1, 0, 0};  // set to c only once its tables are in place
   if (c.block == 0) return U1_OK;
   std::vector<RepDesc> reps(e->upd_host);
   std::vector<uint16_t> pmap;
@@ -691,6 +691,7 @@ int ensemble_build_pair(u1_ensemble* e, Cfg c) {
   HIPCHK(hipMemcpyAsync(e->pair_dev, reps.data(), e->nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
   e->pair_blocks = nb;
+  e->pcfg = c;
   return U1_OK;
 }
 
