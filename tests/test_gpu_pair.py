@@ -187,3 +187,26 @@ def test_ensemble_pairs_fp32_match_unfused(u1):
         u1.tune_enable(True)
     for r in range(len(dims)):
         assert np.array_equal(out[0][r], out[2][r]), r
+
+
+def test_fp64_parity_at_48x4(u1, orc):
+    """The production-size path in binary64 at 48^4: tuned launch shapes
+    (XCD order, plane tiles), then fused pairs forced, two sweeps each, bit
+    for bit against the oracle, with the fused plaquette of the last sweep."""
+    dims = (48, 48, 48, 48)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=4848)
+    m, om = u1.BENCH_METRO, orc.metro(1, 4, 0.5)
+    try:
+        for mode, sweeps in ((1, (0, 1)), (2, (2, 3))):
+            u1.set_pair_fusion(mode)
+            for s in sweeps:
+                say(f"fp64 48^4 parity, pair fusion mode {mode}, sweep {s}")
+                assert lat.update(1, 3, m, 4848, s, count_accepts=True) == \
+                    orc.update(og, ref, 1, 3, om, 4848, s, count_accepts=True), s
+            assert np.array_equal(bits(lat.to_host()), bits(ref)), mode
+        p = lat.update_plaquette(1, 3, m, 4848, 4)
+        orc.update(og, ref, 1, 3, om, 4848, 4)
+    finally:
+        u1.set_pair_fusion(1)
+    assert np.array_equal(bits(lat.to_host()), bits(ref))
+    assert np.max(np.abs(p - orc.plaquette(og, ref))) <= 1e-12
