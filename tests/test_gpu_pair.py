@@ -77,7 +77,7 @@ def test_fp64_pair_masked_and_multilevel(u1, orc, forced):
 
 
 @pytest.mark.parametrize("dims,tune", [((16, 16, 16, 16), False), ((48, 48, 4, 4), False),
-                                       ((48, 48, 4, 4), True), ((24, 24, 8, 8), True)])
+                                       ((48, 48, 4, 4), True), ((24, 24, 8, 8), True), ((48, 48, 48, 48), True)])
 def test_fp32_pair_matches_two_launches(u1, dims, tune):
     """FP32 has no oracle: the fused path is pinned to the two-launch FP32
     path, bit for bit, over sweeps with and without the fused plaquette."""
