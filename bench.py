@@ -15,18 +15,24 @@ so a sweep is 16 V = 84,934,656 link updates.
 
 The JSON line carries:
   value          FP32-bulk link updates/s (whole timed region, inputs resident)
-  roofline       dominant FP32 overrelaxation kernel (the (parity, mu) phase,
-                 or the fused (parity, 0) + (parity, 1) pair where the tuner
-                 chose it, whichever takes more time per sweep): algorithmic
-                 bytes per launch (phase: 4V angles read + V/2 written; pair:
-                 4V read + V written; 4 B each) / mean launch duration from
-                 HIP events on the library stream around 32 back-to-back
-                 launches of that class in the sweep's shapes (on a scratch
-                 copy of the lattice after the timed region), against 8 TB/s;
+  roofline       dominant FP32 kernel: the launch class of the sweep with the
+                 largest time per sweep (Metropolis phase, overrelaxation
+                 phase, measuring phase, or a fused (parity, 0) + (parity, 1)
+                 pair): algorithmic bytes per launch of that class (phase:
+                 4V angles read + V/2 written; pair: 4V read + V written; 4 B
+                 each) / mean launch duration from HIP events on the library
+                 stream around 32 back-to-back launches of that class in the
+                 sweep's shapes (on a scratch copy of the lattice after the
+                 timed region), against 8 TB/s; sweep_frac: the whole sweep's
+                 algorithmic bytes over the timed region, against 8 TB/s;
                  traffic (HBM bytes per launch) from rocprofv3 FETCH_SIZE /
                  WRITE_SIZE counter files when given with --traffic, else null;
                  stream_reference_GBs: plain streaming kernels moving the same
                  bytes (read only, copy, and the phase's 8:1 read:write mix)
+  batched        the 64-replica mix of uneven lattices (8^4 ... 24^3 x 32,
+                 beta ~ U[0.9, 1.1], hot/cold starts), one hipGraph per sweep
+                 with every replica's plaquette summed inside the sweep: FP32
+                 and FP64 aggregate link updates/s, launch classes, roofline
   cpu_baseline   the FP64 host oracle (OpenMP) on a bounded sample of the
                  same sweeps, timed on this host's cores
   fp64 / ratio / plaquette errors as extra fields.
@@ -52,6 +58,34 @@ HBM_PEAK_GBS = 8000.0
 CLASSES = ("metropolis", "overrelaxation", "overrelaxation_with_plaquette", "metropolis_pair",
            "overrelaxation_pair", "overrelaxation_pair_with_plaquette")
 PAIR = (False, False, False, True, True, True)
+KERNEL_NAMES = ("k_phase<float> Metropolis (parity, mu) phase",
+                "k_phase<float> overrelaxation (parity, mu) phase",
+                "k_phase<float, MEAS> overrelaxation phase summing the plaquette",
+                "k_pair<float> fused Metropolis (parity, 0) + (parity, 1)",
+                "k_pair<float> fused overrelaxation (parity, 0) + (parity, 1)",
+                "k_pair<float, MEAS> fused overrelaxation pair summing the plaquette")
+# kernel-trace names of each class (template arguments: phase <Real, MU, KIND,
+# BATCHED, SPT, MEAS>, pair <Real, KIND, SPT, MEAS, BATCHED>; KIND 0 Metropolis,
+# 1 overrelaxation)
+TRACE_PATTERNS = (r"k_phase<float, \d, 0, (false|true), \d, false>",
+                  r"k_phase<float, \d, 1, (false|true), \d, false>",
+                  r"k_phase<float, \d, 1, (false|true), \d, true>",
+                  r"k_pair<float, 0, \d, false(, (false|true))?>",
+                  r"k_pair<float, 1, \d, false(, (false|true))?>",
+                  r"k_pair<float, 1, \d, true(, (false|true))?>")
+
+
+def ensemble_specs(seed=2024, n=64):
+    """The batched-ensemble workload: n replicas drawn by seed from 8^4,
+    12^3 x 16, 16^4, 16^3 x 32 and 24^3 x 32, beta ~ U[0.90, 1.10], hot or
+    cold starts, Philox seeds."""
+    rng = np.random.default_rng(seed)
+    shapes = [(8, 8, 8, 8), (12, 12, 12, 16), (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 32)]
+    dims = [shapes[i] for i in rng.integers(0, len(shapes), n)]
+    betas = rng.uniform(0.90, 1.10, n)
+    hot = rng.integers(0, 2, n)
+    seeds = rng.integers(1, 2**40, n)
+    return dims, betas, hot, seeds
 
 
 def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
@@ -66,26 +100,50 @@ def run_path(u1, prec, dims, beta, seed, warmup, steps, plaq_mode):
 
 
 def class_table(kms, V, s):
-    """Per launch class: mean ms, launches per sweep, algorithmic bytes per launch."""
+    """Per launch class: mean ms, launches per sweep, algorithmic bytes per
+    launch (V: sites the launch covers, over all replicas), achieved GB/s,
+    time per sweep and its share of the sum over classes."""
     out = {}
     for c, name in enumerate(CLASSES):
         if kms[6 + c] > 0:
-            out[name] = {"mean_launch_ms": round(float(kms[c]), 5), "per_sweep": round(float(kms[6 + c]), 3),
-                         "bytes_per_launch": (5.0 if PAIR[c] else 4.5) * V * s}
+            b = (5.0 if PAIR[c] else 4.5) * V * s
+            t = float(kms[c])
+            out[name] = {"mean_launch_ms": round(t, 5), "per_sweep": round(float(kms[6 + c]), 3),
+                         "bytes_per_launch": b, "achieved_GBs": round(b / (t * 1e-3) / 1e9, 1),
+                         "ms_per_sweep": round(t * float(kms[6 + c]), 5)}
+    tot = sum(v["ms_per_sweep"] for v in out.values())
+    for v in out.values():
+        v["time_share"] = round(v["ms_per_sweep"] / tot, 4) if tot > 0 else None
     return out
 
 
-def counter_traffic(fetch_csv, write_csv, pair):
-    """HBM bytes per launch of the FP32 overrelaxation kernel (the plain phase
-    k_phase, or the plain fused pair k_pair) from two rocprofv3 --pmc runs
-    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
-    counter reads for the plain copy kernel k_convert_precision<float, float>
-    (Grid_Size elements of 4 B read and 4 B written) in the same run.
+def roofline_of(cls, sweep_ms):
+    """Roofline object for the launch class with the largest time per sweep,
+    on its own algorithmic bytes, plus the sweep-level fraction."""
+    dom = max(cls, key=lambda k: cls[k]["ms_per_sweep"])
+    d = cls[dom]
+    achieved = d["bytes_per_launch"] / (d["mean_launch_ms"] * 1e-3) / 1e9
+    sweep_bytes = sum(v["bytes_per_launch"] * v["per_sweep"] for v in cls.values())
+    sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    return dom, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                 "kernel": KERNEL_NAMES[CLASSES.index(dom)], "class": dom,
+                 "bytes_per_launch": d["bytes_per_launch"], "mean_launch_ms": d["mean_launch_ms"],
+                 "time_share": d["time_share"], "launch_classes": cls,
+                 "sweep_algorithmic_bytes": sweep_bytes, "sweep_algorithmic_GBs": round(sweep_gbs, 1),
+                 "sweep_frac": round(sweep_gbs / HBM_PEAK_GBS, 4)}
+
+
+def counter_traffic(fetch_csv, write_csv, cls_name):
+    """HBM bytes per launch of the launch class cls_name from two rocprofv3
+    --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what
+    the same counter reads for the plain copy kernel k_convert_precision<float,
+    float> (Grid_Size elements of 4 B read and 4 B written) in the same run.
     Returns (traffic bytes, fetch factor, write factor)."""
     import csv
     import re
 
-    pat = re.compile(r"k_pair<float, 1, \d, false>" if pair else r"k_phase<float, \d, 1, false, \d, false>")
+    pat = re.compile(TRACE_PATTERNS[CLASSES.index(cls_name)])
 
     def scan(path):
         vals, calib = [], []
@@ -97,13 +155,48 @@ def counter_traffic(fetch_csv, write_csv, pair):
                 elif "k_convert_precision<float, float>" in name:
                     calib.append((float(row["Counter_Value"]), 4.0 * float(row["Grid_Size"])))
         if not vals or not calib:
-            raise ValueError(f"{path}: no phase-kernel or calibration dispatches")
+            raise ValueError(f"{path}: no {cls_name} or calibration dispatches")
         kib, nbytes = calib[-1]
         return float(np.mean(vals)), kib * 1024.0 / nbytes
 
     fetch_kib, ff = scan(fetch_csv)
     write_kib, fw = scan(write_csv)
     return fetch_kib * 1024.0 / ff + write_kib * 1024.0 / fw, ff, fw
+
+
+def run_batched(u1, steps, warmup, skip_fp64):
+    """The 64-replica ensemble in FP32 and FP64: graph sweeps with per-sweep
+    plaquettes, timed with HIP events; launch classes over all replicas."""
+    dims, betas, hot, seeds = ensemble_specs(2024)
+    geoms = [u1.geom(d, b) for d, b in zip(dims, betas)]
+    vol = sum(int(np.prod(d)) for d in dims)
+    out = {"replicas": len(dims), "sites": vol, "link_updates_per_sweep": 16.0 * vol,
+           "shapes": {"x".join(map(str, sh)): int(sum(1 for d in dims if d == sh)) for sh in sorted(set(dims))},
+           "beta_range": [round(float(betas.min()), 4), round(float(betas.max()), 4)],
+           "hot_starts": int(np.sum(hot))}
+    for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
+        if prec == u1.F64 and skip_fp64:
+            continue
+        print(f"[bench] batched {name}: {len(dims)} replicas, {warmup} thermalisation + {steps} timed graph sweeps",
+              file=sys.stderr, flush=True)
+        e = u1.Ensemble(geoms, seeds, hot, prec, 1, 3, u1.BENCH_METRO)
+        if warmup > 0:
+            e.sweep(warmup, graph=True)  # thermalisation (also runs the ensemble tuner)
+        e.sweep_plaquette(1)  # builds the measuring graph outside the timed region
+        ms, kms, series = e.bench(steps, kernel_timing=True)
+        s = 4 if prec == u1.F32 else 8
+        cls = class_table(kms, vol, s)
+        _, roof = roofline_of(cls, ms / steps)
+        pm = np.array([[u1.mean_plaquette(r) for r in sw] for sw in series])
+        out[name] = {"link_updates_per_s": 16.0 * vol * steps / (ms * 1e-3), "ms_per_sweep": ms / steps,
+                     "roofline": roof, "mean_plaquette_over_replicas": float(pm[-1].mean())}
+        print(f"[bench] batched {name}: {out[name]['link_updates_per_s']:.4e} link updates/s, {ms / steps:.3f} "
+              f"ms/sweep; dominant {roof['class']} frac {roof['frac']}, sweep frac {roof['sweep_frac']}",
+              file=sys.stderr, flush=True)
+        e.destroy()
+    if "fp64" in out:
+        out["fp32_over_fp64"] = out["fp32"]["link_updates_per_s"] / out["fp64"]["link_updates_per_s"]
+    return out
 
 
 def cpu_baseline(dims, beta, seed, budget_s):
@@ -139,6 +232,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="print every autotuner candidate (stderr)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true", help="leave out the 64-replica batched ensemble")
     ap.add_argument("--plaq", choices=["fused", "separate"], default="fused",
                     help="plaquette every sweep: summed inside the last overrelaxation pass (fused) or by a "
                          "separate pass over the lattice")
@@ -188,7 +282,7 @@ def main():
         s = 4 if prec == u1.F32 else 8
         cls = class_table(kms, V, s)
         res[name] = dict(lat=lat, ms=ms, rate=lu_per_sweep * args.steps / (ms * 1e-3), kms=kms, cls=cls, plaq=mean,
-                         plaq_err=err, tau=tau, elem=s)
+                         plaq_err=err, tau=tau, elem=s, series=series)
         print(f"[bench] {name}: {res[name]['rate']:.4e} link updates/s, {ms / args.steps:.3f} ms/sweep, "
               f"<P> = {mean:.8f} +- {err:.1e}; launches (ms x per sweep): "
               + ", ".join(f"{k} {v['mean_launch_ms']:.4f} x {v['per_sweep']:g}" for k, v in cls.items())
@@ -196,30 +290,14 @@ def main():
 
     f32 = res["fp32"]
     cls = f32["cls"]
-    # dominant memory-bound kernel: the plain overrelaxation phase or the plain
-    # fused overrelaxation pair, whichever takes more time per sweep
-    def total(k):
-        return cls[k]["mean_launch_ms"] * cls[k]["per_sweep"] if k in cls else 0.0
-    dom = "overrelaxation_pair" if total("overrelaxation_pair") > total("overrelaxation") else "overrelaxation"
-    d = cls[dom]
-    achieved = d["bytes_per_launch"] / (d["mean_launch_ms"] * 1e-3) / 1e9
     for k, v in cls.items():
-        v["achieved_GBs"] = round(v["bytes_per_launch"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
         if PAIR[CLASSES.index(k)]:  # two phases' worth of per-phase algorithmic bytes
             v["phase_equivalent_GBs"] = round(9.0 * V * f32["elem"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
-    sweep_bytes = sum(v["bytes_per_launch"] * v["per_sweep"] for v in cls.values())
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": ("k_pair<float> fused overrelaxation (parity, 0) + (parity, 1)" if dom.endswith("pair")
-                           else "k_phase<float> overrelaxation (parity, mu) phase"),
-                "bytes_per_launch": d["bytes_per_launch"], "mean_launch_ms": d["mean_launch_ms"],
-                "launch_classes": cls,
-                "kernel_timing": {"back_to_back_launches_per_class": int(f32["kms"][13]),
-                                  "ms_per_sweep_sum_of_classes": round(sum(v["mean_launch_ms"] * v["per_sweep"]
-                                                                           for v in cls.values()), 4),
-                                  "ms_per_sweep_timed_region": round(f32["ms"] / args.steps, 4)},
-                "sweep_algorithmic_bytes": sweep_bytes,
-                "sweep_algorithmic_GBs": round(sweep_bytes / (f32["ms"] / args.steps * 1e-3) / 1e9, 1)}
+    dom, roofline = roofline_of(cls, f32["ms"] / args.steps)
+    achieved, d = roofline["achieved"], cls[dom]
+    roofline["kernel_timing"] = {"back_to_back_launches_per_class": int(f32["kms"][13]),
+                                 "ms_per_sweep_sum_of_classes": round(sum(v["ms_per_sweep"] for v in cls.values()), 4),
+                                 "ms_per_sweep_timed_region": round(f32["ms"] / args.steps, 4)}
 
     # streaming references moving the same 36 B x V/2 bytes as one FP32 phase
     # with an ideal pattern: what this HBM delivers in practice
@@ -229,7 +307,7 @@ def main():
     roofline["frac_of_read8_write1_stream"] = round(achieved / ref["read8_write1"], 4)
 
     if args.traffic:
-        tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], dom.endswith("pair"))
+        tb, ff, fw = counter_traffic(args.traffic[0], args.traffic[1], dom)
         roofline["traffic"] = round(tb)
         roofline["traffic_over_algorithmic"] = round(tb / d["bytes_per_launch"], 3)
         roofline["traffic_calibration"] = {"fetch_factor": round(ff, 4), "write_factor": round(fw, 4)}
@@ -242,15 +320,34 @@ def main():
         extra["fp64_launch_classes"] = f64["cls"]
         d = f32["plaq"] - f64["plaq"]
         comb = math.hypot(f32["plaq_err"], f64["plaq_err"])
+        thermalised = args.warmup >= 500
         extra["plaquette"] = {"fp32": [f32["plaq"], f32["plaq_err"]], "fp64": [f64["plaq"], f64["plaq_err"]],
                               "tau_int_sweeps": {"fp32": f32["tau"], "fp64": f64["tau"]},
                               "diff": d, "diff_over_combined_err": d / comb if comb > 0 else None,
-                              "within_1sigma": abs(d) <= comb}
-        # fixed configuration: FP32 measurement of the final FP64 configuration
+                              "thermalised": thermalised,
+                              "note": ("one chain per precision after the 500-sweep thermalisation" if thermalised
+                                       else f"unthermalised: {args.warmup} thermalisation sweeps (< 500), the "
+                                            "means still drift from the hot start; not a statistical comparison")}
+        # storage rounding: the final FP64 configuration rounded to binary32
+        # and measured by the FP32 plaquette pass
         p64 = f64["lat"].plaquette()
         p32 = f64["lat"].astype(u1.F32).plaquette()
-        extra["fixed_config_plaquette_rel_err"] = float(abs(u1.mean_plaquette(p32) - u1.mean_plaquette(p64))
-                                                        / abs(u1.mean_plaquette(p64)))
+        extra["fp64_config_in_fp32_plaquette_rel_err"] = float(abs(u1.mean_plaquette(p32) - u1.mean_plaquette(p64))
+                                                               / abs(u1.mean_plaquette(p64)))
+    # the timed quantity: the FP32 sweep's in-sweep plaquette of its final
+    # configuration against the FP64 plaquette of the same links widened to
+    # binary64 (SS, ST and <P>, largest relative difference)
+    pt = np.asarray(f32["series"][-1])
+    pw = f32["lat"].astype(u1.F64).plaquette()
+    errs = [abs(pt[k] - pw[k]) / abs(pw[k]) for k in (0, 2)]
+    errs.append(abs(u1.mean_plaquette(pt) - u1.mean_plaquette(pw)) / abs(u1.mean_plaquette(pw)))
+    extra["fixed_config_plaquette_rel_err"] = float(max(errs))
+    extra["fixed_config_plaquette"] = {"timed_fp32_in_sweep": [float(v) for v in pt],
+                                       "fp64_of_same_links": [float(v) for v in pw],
+                                       "rel_err_ss_st_mean": [float(v) for v in errs]}
+
+    if not args.skip_batched:
+        extra["batched"] = run_batched(u1, args.steps, args.warmup, args.skip_fp64)
 
     if args.tunecache and not os.path.exists(args.tunecache):
         u1.tune_save(args.tunecache)

@@ -184,6 +184,15 @@ int u1_ensemble_sweep(u1_ensemble* e, int32_t nsweeps, int32_t use_graph);
  * u1_plaquette's form.  No counterpart in the reference, which holds one
  * lattice per process and measures with Plaquette (include/plaquette.h:12). */
 int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series);
+/* Timing helper (no counterpart in the reference): u1_ensemble_sweep_plaquette
+ * over nsweeps (>= 1; ovrn >= 1) with elapsed_ms the device time from one HIP
+ * event pair on the library stream around the graph replays (series copies
+ * of all but the last chunk included).  kernel_ms (nullable, 14 doubles) as in
+ * u1_bench_updates, for the ensemble's launches over all replicas: classes
+ * 0 Metropolis, 1 overrelaxation, 2 measuring overrelaxation phase, 3 fused
+ * Metropolis pair, 4 fused overrelaxation pair (5 unused), timed on a scratch
+ * copy of the packed lattice after the timed region. */
+int u1_ensemble_bench(u1_ensemble* e, int32_t nsweeps, double* series, double* elapsed_ms, double* kernel_ms);
 /* out[4*nrep]: per-replica plaquette as u1_plaquette. */
 int u1_ensemble_plaquette(u1_ensemble* e, double* out);
 /* Replica r angles as FP64 host array (4V, EO layout); write sets them. */
@@ -204,6 +213,12 @@ int u1_tune_enable(int32_t on);
  * always.  The tune cache keeps the decision as "MetropolisPair" /
  * "OverRelaxationPair" entries (block.x 0 = two launches). */
 int u1_set_pair_fusion(int32_t mode);
+/* Multilevel level-2 steps (u1_multilevel): replayed from one captured
+ * hipGraph (on = 1, default) or enqueued eagerly (0).  The same kernels read
+ * the same device-side step counter either way, so the results are
+ * identical.  No counterpart in the reference, which launches eagerly
+ * (multilevel.cu:680-700).  Callable before u1_init. */
+int u1_set_graph_capture(int32_t on);
 /* Verbosity (parameters.h:33-37): 0 SILENT, 1 SUMMARIZE (the tuner reports
  * its choice on stderr), 2 VERBOSE and 3 DEBUG_VERBOSE (every candidate's
  * time).  Callable before u1_init. */

@@ -103,6 +103,7 @@ _SIGS = {
                            C.c_int, C.c_int32, C.c_int32, C.POINTER(Metro)),
     "u1_ensemble_sweep": (_P, C.c_int32, C.c_int32),
     "u1_ensemble_sweep_plaquette": (_P, C.c_int32, C.POINTER(C.c_double)),
+    "u1_ensemble_bench": (_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)),
     "u1_ensemble_plaquette": (_P, C.POINTER(C.c_double)),
     "u1_ensemble_read": (_P, C.c_int32, C.POINTER(C.c_double)),
     "u1_ensemble_write": (_P, C.c_int32, C.POINTER(C.c_double)),
@@ -110,6 +111,7 @@ _SIGS = {
     "u1_ensemble_destroy": (_P,),
     "u1_tune_enable": (C.c_int32,),
     "u1_set_pair_fusion": (C.c_int32,),
+    "u1_set_graph_capture": (C.c_int32,),
     "u1_set_verbosity": (C.c_int32,),
     "u1_tune_load": (C.c_char_p,),
     "u1_tune_save": (C.c_char_p,),
@@ -395,6 +397,15 @@ class Ensemble:
         call("u1_ensemble_sweep_plaquette", self.ptr, nsweeps, _dp(out))
         return out.reshape(nsweeps, self.n, 4)
 
+    def bench(self, nsweeps: int, kernel_timing: bool = False):
+        """sweep_plaquette timed with HIP events on the library stream: returns
+        (ms, kernel_ms or None, series); kernel_ms as Lattice.bench."""
+        out = np.zeros(4 * self.n * nsweeps)
+        ms = C.c_double(0.0)
+        kms = np.zeros(14) if kernel_timing else None
+        call("u1_ensemble_bench", self.ptr, nsweeps, _dp(out), C.byref(ms), _dp(kms) if kernel_timing else None)
+        return ms.value, kms, out.reshape(nsweeps, self.n, 4)
+
     def plaquette(self) -> np.ndarray:
         out = np.zeros(4 * self.n)
         call("u1_ensemble_plaquette", self.ptr, _dp(out))
@@ -456,6 +467,12 @@ def set_verbosity(level: int) -> None:
 
 def tune_enable(on: bool) -> None:
     call("u1_tune_enable", 1 if on else 0)
+
+
+def set_graph_capture(on: bool) -> None:
+    """Multilevel level-2 steps from one captured hipGraph (True, default) or
+    enqueued eagerly (False); identical results."""
+    call("u1_set_graph_capture", 1 if on else 0)
 
 
 def set_pair_fusion(mode: int) -> None:

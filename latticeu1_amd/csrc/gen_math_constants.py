@@ -6,9 +6,12 @@ module at 100 significant digits and rounded once to binary64 / binary32:
 
 * pi from Machin's formula  pi = 16 atan(1/5) - 4 atan(1/239);
 * ln 2 from the series      ln 2 = 2 atanh(1/3);
-* Taylor coefficients 1/n! and 1/(2k+1) exactly as rationals;
-* one minimax table, binary32 atan on [0, 1] for the FP32 update kernels,
-  from a Remez exchange (remez_atan01), kept as literals.
+* Taylor coefficients 1/n! and 1/(2k+1) exactly as rationals (binary32
+  sin/cos/atan of the measurement kernels);
+* minimax tables from Remez exchanges, kept as literals: binary32 atan on
+  [0, 1] for the FP32 update kernels (remez_atan01), and the binary64 sin,
+  cos, exp and atan polynomials shared by the oracle and the FP64 kernels
+  (remez_mp, in 60-digit mpmath arithmetic).
 
 Cody-Waite splits (pi/2 = P1 + P2 + P3, ln2 = HI + LO) take the remainder of
 the exact value after subtracting the exactly-represented leading parts, so the
@@ -148,14 +151,11 @@ def inv_fact(n: int) -> Fraction:
 def coeff_tables():
     """Coefficient tables, highest order first (Horner order)."""
     t = {}
-    # sin r = r + r^3 * P(r^2), P has terms (-1)^k / (2k+3)!, k = 0..7  (through r^17)
-    t["SIN64"] = [to_f64(inv_fact(2 * k + 3) * (-1) ** (k + 1)) for k in range(7, -1, -1)]
-    # cos r = 1 + r^2 * Q(r^2), Q has (-1)^(k+1)/(2k+2)!, k = 0..7 (through r^16)
-    t["COS64"] = [to_f64(inv_fact(2 * k + 2) * (-1) ** (k + 1)) for k in range(7, -1, -1)]
-    # exp r = 1 + r + r^2 * E(r), E has 1/(k+2)!, k = 0..11 (through r^13)
-    t["EXP64"] = [to_f64(inv_fact(k + 2)) for k in range(11, -1, -1)]
-    # atan t = t + t^3 * A(t^2), A has (-1)^(k+1)/(2k+3), k = 0..19 (through t^41)
-    t["ATAN64"] = [to_f64(Fraction((-1) ** (k + 1), 2 * k + 3)) for k in range(19, -1, -1)]
+    # binary64 minimax (remez_mp): sin r = r + r^3 P(r^2) through r^13,
+    # cos r = 1 + r^2 Q(r^2) through r^14, e^r = 1 + r + r^2 E(r) through r^12,
+    # atan t = t + t^3 A(t^2) through t^23
+    for name in MM64:
+        t[name] = [float.fromhex(h) for h in MM64_HEX[name]]
     # float tables
     t["SIN32"] = [to_f32(inv_fact(2 * k + 3) * (-1) ** (k + 1)) for k in range(3, -1, -1)]  # through r^9
     t["COS32"] = [to_f32(inv_fact(2 * k + 2) * (-1) ** (k + 1)) for k in range(4, -1, -1)]  # through r^10
@@ -173,6 +173,116 @@ def coeff_tables():
 # (the test suite re-measures it).
 ATAN01_MM32_HEX = ["0x1.7ed232p-9", "-0x1.0c2c14p-6", "0x1.61fddap-5", "-0x1.3556b6p-4",
                    "0x1.b4e12ap-4", "-0x1.230adcp-3", "0x1.9978f4p-3", "-0x1.5554dcp-2"]
+
+
+# Binary64 minimax tables (highest order first): the output of remez_mp(name,
+# n) rounded once to binary64, kept as literals.  Weighted error (relative
+# error of the function; sin, cos on |r| <= pi/4, e^r on |r| <= ln2/2, atan on
+# |t| <= tan(pi/8)) after rounding, in units of 2^-53: sin 0.10, cos 0.039,
+# exp 0.0064, atan 0.039 (the test suite re-measures them and re-runs the
+# exchange).
+MM64 = {"SIN64": 6, "COS64": 7, "EXP64": 11, "ATAN64": 11}
+MM64_HEX = {
+    "SIN64": ["0x1.5d8ff90144cdcp-33", "-0x1.ae5e5b1533e8ep-26", "0x1.71de356876082p-19", "-0x1.a01a019bfea3dp-13",
+              "0x1.111111110f7d6p-7", "-0x1.5555555555548p-3"],
+    "COS64": ["-0x1.8f7571988e6d6p-37", "0x1.1ee96d22197a9p-29", "-0x1.27e4f72c0286ep-22", "0x1.a01a019b2e602p-16",
+              "-0x1.6c16c16c13bb1p-10", "0x1.5555555555539p-5", "-0x1.0000000000000p-1"],
+    "EXP64": ["0x1.1e3d557e25f52p-29", "0x1.af6bad920703bp-26", "0x1.27e5b3f37a4fbp-22", "0x1.71ddf0419bf66p-19",
+              "0x1.a01a0136d60b9p-16", "0x1.a01a01b3715f4p-13", "0x1.6c16c16c30321p-10", "0x1.111111110ec7ep-7",
+              "0x1.5555555555503p-5", "0x1.555555555555bp-3", "0x1.0000000000000p-1"],
+    "ATAN64": ["-0x1.2565a607b5743p-6", "0x1.37d2c7d80af0ap-5", "-0x1.9cd14a1039218p-5", "0x1.df0ee3b60d900p-5",
+               "-0x1.10ebaf736a466p-4", "0x1.3b11182728061p-4", "-0x1.745cf8cb68c93p-4", "0x1.c71c70e552a3ap-4",
+               "-0x1.24924922aaf63p-3", "0x1.99999999957e7p-3", "-0x1.555555555553dp-2"],
+}
+
+
+def _mm64_problem(name):
+    """(g, W, a, b) of a binary64 minimax table: its polynomial p minimises
+    max |W(x) (g(x) - p(x))| over [a, b]; W (g - p) is the relative error of
+    the function the table builds."""
+    import mpmath as mp
+
+    def nz(x):  # the quotients below at x = 0 are limits; evaluate just beside it
+        return x if x != 0 else mp.mpf(10) ** -40
+
+    if name == "SIN64":  # sin r = r + r^3 P(r^2), w = r^2 <= (pi/4)^2
+        return (lambda w: (mp.sin(mp.sqrt(nz(w))) - mp.sqrt(nz(w))) / nz(w) ** mp.mpf(1.5),
+                lambda w: nz(w) ** mp.mpf(1.5) / mp.sin(mp.sqrt(nz(w))), mp.mpf(0), (mp.pi / 4) ** 2)
+    if name == "COS64":  # cos r = 1 + r^2 Q(r^2)
+        return (lambda w: (mp.cos(mp.sqrt(nz(w))) - 1) / nz(w), lambda w: nz(w) / mp.cos(mp.sqrt(nz(w))),
+                mp.mpf(0), (mp.pi / 4) ** 2)
+    if name == "EXP64":  # e^r = 1 + r + r^2 E(r), |r| <= ln2 / 2
+        return (lambda r: (mp.exp(nz(r)) - 1 - nz(r)) / nz(r) ** 2, lambda r: nz(r) ** 2 / mp.exp(nz(r)),
+                -mp.log(2) / 2, mp.log(2) / 2)
+    if name == "ATAN64":  # atan t = t + t^3 A(t^2), w = t^2 <= tan(pi/8)^2
+        return (lambda w: (mp.atan(mp.sqrt(nz(w))) - mp.sqrt(nz(w))) / nz(w) ** mp.mpf(1.5),
+                lambda w: nz(w) ** mp.mpf(1.5) / mp.atan(mp.sqrt(nz(w))), mp.mpf(0), (mp.sqrt(2) - 1) ** 2)
+    raise KeyError(name)
+
+
+def remez_mp(name, n, iters=10, grid=3000):
+    """Remez exchange (Remez 1934) in 60-digit arithmetic (mpmath) for the n
+    coefficients of table `name` (see _mm64_problem): the reference set starts
+    at the Chebyshev extrema and moves to the n + 1 alternating extrema of the
+    weighted error on a grid.  Returns (coefficients lowest order first as
+    binary64, levelled error)."""
+    import mpmath as mp
+
+    g, W, a, b = _mm64_problem(name)
+    with mp.workdps(60):
+        ref = [(a + b) / 2 - (b - a) / 2 * mp.cos(mp.pi * k / n) for k in range(n + 1)]
+        ref = [x + (b - a) / 1000 if abs(x) < (b - a) / 10**6 else x for x in ref]  # W vanishes at 0
+        xs = [a + (b - a) * (i + mp.mpf(0.5)) / (grid + 1) for i in range(grid + 1)]
+        xs = [x for x in xs if abs(x) > (b - a) / 10**4]
+        gx = [g(x) for x in xs]
+        wx = [W(x) for x in xs]
+        c, lev = None, None
+        for _ in range(iters):
+            A = mp.matrix(n + 1, n + 1)
+            rhs = mp.matrix(n + 1, 1)
+            for i, x in enumerate(ref):
+                for j in range(n):
+                    A[i, j] = x**j
+                A[i, n] = (-1) ** i / W(x)
+                rhs[i] = g(x)
+            sol = mp.lu_solve(A, rhs)
+            c = [sol[j] for j in range(n)]
+            lev = abs(sol[n])
+            e = []
+            for x, gv, wv in zip(xs, gx, wx):
+                p = mp.mpf(0)
+                for cc in reversed(c):
+                    p = p * x + cc
+                e.append(wv * (gv - p))
+            idx, start = [], 0  # the largest |error| of each run of equal sign
+            for i in range(1, len(e) + 1):
+                if i == len(e) or mp.sign(e[i]) != mp.sign(e[start]):
+                    idx.append(max(range(start, i), key=lambda t: abs(e[t])))
+                    start = i
+            while len(idx) > n + 1:
+                idx = idx[1:] if abs(e[idx[0]]) < abs(e[idx[-1]]) else idx[:-1]
+            if len(idx) < n + 1:
+                break
+            ref = [xs[k] for k in idx]
+        return [float(v) for v in c], float(lev)
+
+
+def mm64_table_error(name, npts=2000):
+    """Largest weighted error of the committed binary64 table, evaluated
+    exactly (60 digits) at npts points of its interval."""
+    import mpmath as mp
+
+    g, W, a, b = _mm64_problem(name)
+    coeffs = [float.fromhex(h) for h in reversed(MM64_HEX[name])]
+    with mp.workdps(60):
+        worst = mp.mpf(0)
+        for i in range(npts):
+            x = a + (b - a) * (i + mp.mpf(0.5)) / npts
+            p = mp.mpf(0)
+            for cc in reversed(coeffs):
+                p = p * x + mp.mpf(cc)
+            worst = max(worst, abs(W(x) * (g(x) - p)))
+        return float(worst)
 
 
 def _atan01_q(w):
@@ -235,7 +345,8 @@ def render() -> str:
     out.append("// Generated by gen_math_constants.py -- do not edit by hand.")
     out.append("// Each value is the correctly rounded binary64/binary32 image of an exact")
     out.append("// mathematical constant computed with 100-digit decimal arithmetic, except")
-    out.append("// the ATAN01_MM32 table (minimax coefficients from a Remez exchange).")
+    out.append("// the minimax tables SIN64, COS64, EXP64, ATAN64 and ATAN01_MM32 (Remez")
+    out.append("// exchanges).")
     out.append("#pragma once")
     out.append("")
     out.append("namespace u1m {")

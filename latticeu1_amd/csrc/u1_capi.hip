@@ -43,6 +43,9 @@ struct Ctx {
   // 0 never, 1 where the tuner measures them faster, 2 always
   int pair = 1;
   int verbosity = 0;                      // 0 silent, 1 summarize, 2 verbose (tuner output on stderr)
+  // multilevel level-2 steps: replayed from one captured hipGraph (true) or
+  // enqueued eagerly (false); the same kernels and counters either way
+  bool graphs = true;
   std::map<std::string, Cfg> tunecache;  // key -> launch shape
   int tune_loaded = 0;
 };
@@ -178,6 +181,7 @@ size_t elem_size(int prec) { return prec == U1_F64 ? sizeof(double) : sizeof(flo
 
 int ensure_partials(size_t n) {
   if (n <= G.partials_cap) return U1_OK;
+  HIPCHK(hipStreamSynchronize(G.st));  // no queued kernel still reads the old buffer
   if (G.partials) HIPCHK(hipFree(G.partials));
   G.partials = nullptr;
   size_t cap = std::max<size_t>(n, 1 << 16);
@@ -190,6 +194,7 @@ int ensure_partials(size_t n) {
 // per call, polyakov.cu:443-508, 618-654).
 int ensure_work(size_t n) {
   if (n <= G.work_cap) return U1_OK;
+  HIPCHK(hipStreamSynchronize(G.st));
   if (G.work) HIPCHK(hipFree(G.work));
   G.work = nullptr;
   G.work_cap = 0;
@@ -334,10 +339,10 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
 // bound (larger planes take more steps).
 std::vector<Cfg> pair_candidates(int prec, const RepDesc& r) {
   const int P = r.nxh * r.N[1];
-  const int maxb = prec == U1_F64 ? 512 : 1024;
   std::vector<Cfg> v;
-  for (int spt : {1, 2, 4}) {
-    if (spt == 4 && prec != U1_F32) continue;
+  for (int spt : {1, 2, 3, 4}) {
+    if (spt >= 3 && prec != U1_F32) continue;
+    const int maxb = u1d::pair_max_block((int)elem_size(prec), spt);
     int prev = 0;
     for (int steps = 1; steps <= 2; ++steps) {
       const int b = std::min(maxb, ((P + spt * steps - 1) / (spt * steps) + 63) / 64 * 64);
@@ -350,9 +355,9 @@ std::vector<Cfg> pair_candidates(int prec, const RepDesc& r) {
   return v;
 }
 bool pair_shape_ok(int prec, const Cfg& c) {
-  const int maxb = prec == U1_F64 ? 512 : 1024;
-  return c.block >= 64 && c.block <= maxb && c.block % 64 == 0 &&
-         (c.spt == 1 || c.spt == 2 || (c.spt == 4 && prec == U1_F32));
+  if (!(c.spt == 1 || c.spt == 2 || ((c.spt == 3 || c.spt == 4) && prec == U1_F32))) return false;
+  const int maxb = u1d::pair_max_block((int)elem_size(prec), c.spt);
+  return c.block >= 64 && c.block <= maxb && c.block % 64 == 0;
 }
 // Plane decomposition of the fused pair: one chunk (workgroup) per plane.
 void set_pair_blocks(RepDesc& r, const Cfg& c) {
@@ -830,6 +835,142 @@ void ensemble_release(u1_ensemble* e) {
   if (e->partials) (void)hipFree(e->partials);
 }
 
+// The measuring sweep graph (once per ensemble): phases with the plaquette
+// summed in the last pass, the per-replica fold into the series row of the
+// sweep, sweep index + 1.
+int ensemble_build_mgraph(u1_ensemble* e) {
+  if (e->mexec) return U1_OK;
+  // (kept across a failed capture and reused by the next attempt, unless a
+  // later tuning has given the update launches more workgroups)
+  if (e->mrows && e->mrows_blocks < e->upd_blocks) {
+    HIPCHK(hipFree(e->mrows));
+    e->mrows = nullptr;
+  }
+  if (!e->mrows) {
+    HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
+    e->mrows_blocks = e->upd_blocks;
+  }
+  if (!e->series_dev)
+    HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
+  if (!e->base_dev) HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
+  HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
+  int rc = ensemble_enqueue_phases(e, e->lat, e->mrows);
+  if (rc == U1_OK) {
+    hipError_t f = u1d::launch_ensemble_plaquette_fold(e->mrows, 3, e->upd_blocks, e->upd_dev, e->nrep, e->sweep_dev,
+                                                       e->base_dev, kEnsSeriesCap, e->series_dev, G.st);
+    if (f == hipSuccess) f = u1d::launch_increment(e->sweep_dev, G.st);
+    if (f != hipSuccess) rc = fail(U1_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(f));
+  }
+  hipGraph_t gr = nullptr;
+  const hipError_t ce = hipStreamEndCapture(G.st, &gr);
+  if (rc != U1_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  HIPCHK(ce);
+  e->mgraph = gr;
+  HIPCHK(hipGraphInstantiate(&e->mexec, e->mgraph, nullptr, nullptr, 0));
+  return U1_OK;
+}
+
+// nsweeps replays of the measuring graph, the series copied to the host in
+// chunks of kEnsSeriesCap sweeps.  done_ev (optional) is recorded on the
+// library stream after the last sweep's work, before the last chunk's copy.
+int ensemble_measured_sweeps(u1_ensemble* e, int32_t nsweeps, double* series, hipEvent_t done_ev = nullptr) {
+  for (int32_t done = 0; done < nsweeps;) {
+    const int32_t n = std::min<int32_t>(kEnsSeriesCap, nsweeps - done);
+    HIPCHK(hipMemcpyAsync(e->base_dev, e->sweep_dev, sizeof(uint32_t), hipMemcpyDeviceToDevice, G.st));
+    for (int32_t s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(e->mexec, G.st));
+    if (done_ev && done + n == nsweeps) HIPCHK(hipEventRecord(done_ev, G.st));
+    HIPCHK(hipMemcpyAsync(series + 4 * (size_t)e->nrep * done, e->series_dev, 4 * (size_t)e->nrep * n * sizeof(double),
+                          hipMemcpyDeviceToHost, G.st));
+    HIPCHK(hipStreamSynchronize(G.st));
+    done += n;
+    e->sweeps_done += (uint64_t)n;
+  }
+  return U1_OK;
+}
+
+// Launch classes of a sweep, for kernel timing.
+enum LaunchClass { LC_METRO = 0, LC_OR, LC_OR_MEAS, LC_METRO_PAIR, LC_OR_PAIR, LC_OR_PAIR_MEAS, LC_COUNT };
+constexpr int kTimingLaunches = 32;  // back-to-back launches per class
+
+// Kernel timing of the ensemble's launch classes (shapes settled, ovrn >= 1):
+// on a scratch copy of the packed lattice, kTimingLaunches back-to-back
+// launches of each class over all replicas, in the sweep's launch tables, with
+// (parity, mu) cycling over the phases the class covers in a pass.  ms[c]:
+// mean launch duration; per_sweep[c]: launches per measuring sweep (the last
+// overrelaxation pass measures in its parity-1 launches of mu >= 1 and, with
+// fused pairs, leaves its parity-1 pair unfused).
+int time_ensemble_classes(u1_ensemble* e, double* ms, double* per_sweep) {
+  const size_t bytes = (size_t)e->total_links * elem_size(e->prec);
+  DevBuf scratch, rows;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(rows.alloc(4 * (size_t)e->upd_blocks * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(scratch.p, e->lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  EventSet es;
+  hipEvent_t e0, e1;
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
+  PhaseArgs a{};
+  a.reps = e->upd_dev;
+  a.blockmap = e->upd_map;
+  a.proposal = e->m.proposal;
+  a.nhit = e->m.nhit;
+  a.step = e->m.step;
+  a.stream = u1r::STREAM_UPDATE;
+  a.sweep = (uint32_t)e->sweeps_done;
+  a.xcd = (uint32_t)e->cfg.xcd;
+  const bool fused = e->pcfg.block != 0;
+  for (int c = 0; c < LC_COUNT; ++c) ms[c] = per_sweep[c] = 0.0;
+  for (int c = 0; c < LC_COUNT; ++c) {
+    if (c == LC_OR_PAIR_MEAS) continue;  // the measuring parity-1 pair stays unfused
+    const bool pair = c == LC_METRO_PAIR || c == LC_OR_PAIR;
+    const int kind = (c == LC_METRO || c == LC_METRO_PAIR) ? u1d::KIND_METROPOLIS : u1d::KIND_OVERRELAX;
+    const int passes = kind == u1d::KIND_METROPOLIS ? e->metrop : e->ovrn;
+    const bool cmeas = c == LC_OR_MEAS, measured_kind = kind == u1d::KIND_OVERRELAX;
+    if (passes == 0 || (pair && !fused)) continue;
+    std::vector<std::pair<int, int>> ph;  // (parity, mu); a pair is listed by mu 0
+    for (int parity = 0; parity < 2; ++parity)
+      for (int mu = 0; mu < 4; ++mu) {
+        if (cmeas ? !(parity == 1 && mu >= 1) : (pair ? mu != 0 : (fused && mu < 2))) continue;
+        ph.push_back({parity, mu});
+      }
+    if (cmeas)
+      per_sweep[c] = 3;
+    else if (pair)
+      per_sweep[c] = 2.0 * passes - (measured_kind ? 1 : 0);
+    else
+      per_sweep[c] = (fused ? 4.0 : 8.0) * passes - (measured_kind ? (fused ? 1 : 3) : 0);
+    PhaseArgs b = a;
+    if (pair) {
+      b.reps = e->pair_dev;
+      b.blockmap = e->pair_map;
+      b.xcd = (uint32_t)e->pcfg.xcd;
+    }
+    b.meas = cmeas ? rows.as<double>() : nullptr;
+    auto launch = [&](int i) -> int {
+      b.parity = ph[(size_t)i % ph.size()].first;
+      b.pass = (uint32_t)(i / (int)ph.size());
+      if (pair)
+        HIPCHK(u1d::launch_pair(e->prec, kind, scratch.p, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt, G.st));
+      else
+        HIPCHK(u1d::launch_phase(e->prec, kind, ph[(size_t)i % ph.size()].second, scratch.p, b, e->upd_blocks,
+                                 e->cfg.block, e->cfg.spt, G.st));
+      return U1_OK;
+    };
+    RET(launch(0));  // warm-up
+    HIPCHK(hipEventRecord(e0, G.st));
+    for (int i = 0; i < kTimingLaunches; ++i) RET(launch(i));
+    HIPCHK(hipEventRecord(e1, G.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    ms[c] = t / kTimingLaunches;
+  }
+  return U1_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -843,7 +984,11 @@ int u1_init(int device) {
   if (n == 0) return fail(U1_ERR_HIP, "no HIP device available");
   if (device < 0 || device >= n) return fail(U1_ERR_ARG, "GPU id out of range");
   HIPCHK(hipSetDevice(device));
-  HIPCHK(hipStreamCreate(&G.st));
+  // non-blocking: no implicit synchronisation with the legacy default
+  // stream, so default-stream work of other code in the process (another
+  // thread, a profiler, a framework) neither waits on the library's launches
+  // nor lands inside one of its stream captures
+  HIPCHK(hipStreamCreateWithFlags(&G.st, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void**)&G.counter, sizeof(unsigned long long)));
   HIPCHK(hipMalloc((void**)&G.dres, 64 * sizeof(double)));
   G.device = device;
@@ -1163,34 +1308,43 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   // One level-2 step -- k2 masked sweeps (Philox sweep index ctr * k2 + l),
   // the temporal-link field (c1 = ctr), the T2 accumulation and ctr += 1 --
   // captured once and replayed n4 * n2 times (multilevel.cu:680-700 order;
-  // sweep and c1 indices equal to ((i n2 + k) k2 + l) and (i n2 + k)).
-  MLCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
-  for (int l = 0; l < k2 && rc == U1_OK; ++l)
-    rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)l, call_id, 2, nullptr, nullptr,
-                    ctr, (uint32_t)k2);
-  if (rc == U1_OK) {
-    const hipError_t e1 = u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT, 0,
-                                                     call_id, (double*)tl, 256, G.st, ctr);
-    const hipError_t e2 = e1 == hipSuccess ? u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st) : e1;
-    const hipError_t e3 = e2 == hipSuccess ? u1d::launch_increment(ctr, G.st) : e2;
-    if (e3 != hipSuccess) rc = fail(U1_ERR_HIP, std::string("multilevel capture: ") + hipGetErrorString(e3));
-  }
-  {
+  // sweep and c1 indices equal to ((i n2 + k) k2 + l) and (i n2 + k)), or
+  // enqueued eagerly with the same device counter when graphs are off.
+  auto level2_step = [&]() -> int {
+    for (int l = 0; l < k2; ++l)
+      RET(run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L2, (uint32_t)l, call_id, 2, nullptr,
+                     nullptr, ctr, (uint32_t)k2));
+    hipError_t e = u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT, 0, call_id,
+                                              (double*)tl, 256, G.st, ctr);
+    if (e == hipSuccess) e = u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st);
+    if (e == hipSuccess) e = u1d::launch_increment(ctr, G.st);
+    if (e != hipSuccess) return fail(U1_ERR_HIP, std::string("multilevel level-2 step: ") + hipGetErrorString(e));
+    return U1_OK;
+  };
+  if (G.graphs) {
+    MLCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
+    rc = level2_step();
     const hipError_t ce = hipStreamEndCapture(G.st, &graph);
     if (rc != U1_OK) {
       cleanup();
       return rc;
     }
     MLCHK(ce);
+    MLCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  MLCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   for (int i = 0; i < n4 && rc == U1_OK; ++i) {
     for (int j = 0; j < k4 && rc == U1_OK; ++j)
       rc = run_update(g, prec, wl, metrop, ovrn, m, seed, u1r::STREAM_ML_L4, (uint32_t)(i * k4 + j), call_id, 4,
                       nullptr);
     if (rc != U1_OK) break;
     MLCHK(u1d::launch_fill_zero(l2, 2 * sz * nl2 * (int64_t)sizeof(double), G.st));
-    for (int k = 0; k < n2; ++k) MLCHK(hipGraphLaunch(exec, G.st));
+    for (int k = 0; k < n2 && rc == U1_OK; ++k) {
+      if (exec)
+        MLCHK(hipGraphLaunch(exec, G.st));
+      else
+        rc = level2_step();
+    }
+    if (rc != U1_OK) break;
     MLCHK(u1d::launch_ml_l4((const double*)l2, (double*)l4, r, rmax, l2norm, 256, G.st));
   }
   if (rc == U1_OK) {
@@ -1425,49 +1579,31 @@ int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series)
   if (e->metrop + e->ovrn == 0) return fail(U1_ERR_ARG, "no measuring pass");
   if (nsweeps == 0) return U1_OK;
   RET(ensemble_settle(e));
-  if (!e->mexec) {
-    // (kept across a failed capture and reused by the next attempt, unless a
-    // later tuning has given the update launches more workgroups)
-    if (e->mrows && e->mrows_blocks < e->upd_blocks) {
-      HIPCHK(hipFree(e->mrows));
-      e->mrows = nullptr;
-    }
-    if (!e->mrows) {
-      HIPCHK(hipMalloc((void**)&e->mrows, 3 * 4 * (size_t)e->upd_blocks * sizeof(double)));
-      e->mrows_blocks = e->upd_blocks;
-    }
-    if (!e->series_dev)
-      HIPCHK(hipMalloc((void**)&e->series_dev, (size_t)kEnsSeriesCap * e->nrep * 4 * sizeof(double)));
-    if (!e->base_dev) HIPCHK(hipMalloc((void**)&e->base_dev, sizeof(uint32_t)));
-    // one sweep: phases (measuring in the last pass), per-replica fold into
-    // the series row of this sweep, sweep index + 1
-    HIPCHK(hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal));
-    int rc = ensemble_enqueue_phases(e, e->lat, e->mrows);
-    if (rc == U1_OK) {
-      hipError_t f = u1d::launch_ensemble_plaquette_fold(e->mrows, 3, e->upd_blocks, e->upd_dev, e->nrep, e->sweep_dev,
-                                                         e->base_dev, kEnsSeriesCap, e->series_dev, G.st);
-      if (f == hipSuccess) f = u1d::launch_increment(e->sweep_dev, G.st);
-      if (f != hipSuccess) rc = fail(U1_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(f));
-    }
-    hipGraph_t gr = nullptr;
-    const hipError_t ce = hipStreamEndCapture(G.st, &gr);
-    if (rc != U1_OK) {
-      if (gr) (void)hipGraphDestroy(gr);
-      return rc;
-    }
-    HIPCHK(ce);
-    e->mgraph = gr;
-    HIPCHK(hipGraphInstantiate(&e->mexec, e->mgraph, nullptr, nullptr, 0));
-  }
-  for (int32_t done = 0; done < nsweeps;) {
-    const int32_t n = std::min<int32_t>(kEnsSeriesCap, nsweeps - done);
-    HIPCHK(hipMemcpyAsync(e->base_dev, e->sweep_dev, sizeof(uint32_t), hipMemcpyDeviceToDevice, G.st));
-    for (int32_t s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(e->mexec, G.st));
-    HIPCHK(hipMemcpyAsync(series + 4 * (size_t)e->nrep * done, e->series_dev, 4 * (size_t)e->nrep * n * sizeof(double),
-                          hipMemcpyDeviceToHost, G.st));
-    HIPCHK(hipStreamSynchronize(G.st));
-    done += n;
-    e->sweeps_done += (uint64_t)n;
+  RET(ensemble_build_mgraph(e));
+  return ensemble_measured_sweeps(e, nsweeps, series);
+}
+
+int u1_ensemble_bench(u1_ensemble* e, int32_t nsweeps, double* series, double* elapsed_ms, double* kernel_ms) {
+  REQ_INIT();
+  if (!e || !series || !elapsed_ms) return fail(U1_ERR_ARG, "null argument");
+  if (nsweeps < 1) return fail(U1_ERR_ARG, "nsweeps must be >= 1");
+  if (e->ovrn < 1) return fail(U1_ERR_ARG, "the ensemble timing needs an overrelaxation pass (ovrn >= 1)");
+  RET(ensemble_settle(e));
+  RET(ensemble_build_mgraph(e));
+  EventSet es;
+  hipEvent_t e0, e1;
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
+  HIPCHK(hipStreamSynchronize(G.st));
+  HIPCHK(hipEventRecord(e0, G.st));
+  RET(ensemble_measured_sweeps(e, nsweeps, series, e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *elapsed_ms = ms;
+  if (kernel_ms) {
+    RET(time_ensemble_classes(e, kernel_ms, kernel_ms + LC_COUNT));
+    kernel_ms[2 * LC_COUNT] = (double)ms / nsweeps;
+    kernel_ms[2 * LC_COUNT + 1] = kTimingLaunches;
   }
   return U1_OK;
 }
@@ -1533,6 +1669,12 @@ int u1_ensemble_destroy(u1_ensemble* e) {
 // ---------------------------------------------------------------- autotuner
 int u1_tune_enable(int32_t on) {
   G.tune = on != 0;
+  return U1_OK;
+}
+
+int u1_set_graph_capture(int32_t on) {
+  if (on != 0 && on != 1) return fail(U1_ERR_ARG, "graph capture switch must be 0 or 1");
+  G.graphs = on != 0;
   return U1_OK;
 }
 
@@ -1619,9 +1761,6 @@ int u1_tune_entries(int32_t* n) {
 }
 
 // ---------------------------------------------------------------- timing helper
-// Launch classes of a sweep, for kernel timing.
-enum LaunchClass { LC_METRO = 0, LC_OR, LC_OR_MEAS, LC_METRO_PAIR, LC_OR_PAIR, LC_OR_PAIR_MEAS, LC_COUNT };
-constexpr int kTimingLaunches = 32;  // back-to-back launches per class
 
 // Kernel timing of the launch classes of a sweep (1 <= metrop + ovrn, shapes
 // already settled): on a scratch copy of the lattice, kTimingLaunches
@@ -1753,7 +1892,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
     kernel_ms[2 * LC_COUNT + 1] = kTimingLaunches;
   }
   if (measure_plaq && plaq_series)
-    HIPCHK(hipMemcpy(plaq_series, series, 4 * (size_t)nsweeps * sizeof(double), hipMemcpyDeviceToHost));
+    RET(copy_back(plaq_series, series, 4 * (size_t)nsweeps));
   return U1_OK;
 }
 
@@ -1806,6 +1945,7 @@ int u1_selftest_stream(int32_t pattern, int64_t n, int32_t reps, double* gbs) {
   if (e != hipSuccess) rc = fail(U1_ERR_HIP, std::string("stream probe: ") + hipGetErrorString(e));
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(G.st);
   (void)hipFree(a);
   (void)hipFree(b);
   return rc;

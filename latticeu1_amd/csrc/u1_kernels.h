@@ -43,9 +43,10 @@ hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseA
 // set_phase_blocks(r, Nx/2 * Ny) and nblocks = Nz * Nt.  Batched replicas
 // (a.reps, a.blockmap): every replica descriptor carries its own
 // set_phase_blocks(r, Nx/2 * Ny) with blk_begin its first workgroup, and the
-// block map covers nblocks; no measurement.  block a multiple of 64 up to
-// 1024 (binary32) or 512 (binary64).  a.meas makes phase (parity, 1)
-// measuring.
+// block map covers nblocks; no measurement.  spt 1, 2 (both precisions), 3 or
+// 4 (binary32); block a multiple of 64 up to pair_max_block.  a.meas makes
+// phase (parity, 1) measuring.
+constexpr int pair_max_block(int elem_bytes, int spt) { return elem_bytes == 8 || spt >= 3 ? 512 : 1024; }
 hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
                        hipStream_t st);
 

@@ -409,12 +409,11 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 // __syncthreads orders the workgroup's global stores before its later loads.
 // The pair reads the lattice once (4V angles) and writes V/2 + V/2: 5V angles
 // against 9V for two launches.  MEAS applies to phase (parity, 1).
-// When one step covers the plane (P <= SPT * blockDim.x, SPT <= 2), every
+// When one step covers the plane (P <= SPT * blockDim.x, SPT <= 3), every
 // load of both phases except the two phase-0 results is issued before phase 0
 // computes, so phase 1 waits on the barrier and two cache-hit loads only.
 template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
-__global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_pair(Real* __restrict__ lat_base,
-                                                                                     PhaseArgs a) {
+__global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real* __restrict__ lat_base, PhaseArgs a) {
   const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
   const uint32_t b = L - (uint32_t)r.blk_begin;  // the replica's plane (one chunk per plane)
@@ -422,7 +421,7 @@ __global__ __launch_bounds__(sizeof(Real) == sizeof(float) ? 1024 : 512) void k_
   const uint32_t step = SPT * blockDim.x;
   double v0[2] = {0.0, 0.0}, v[2] = {0.0, 0.0};
   uint32_t acc0 = 0, acc1 = 0;
-  if (SPT <= 2 && P <= step) {
+  if (SPT <= 3 && P <= step) {
     PhaseIn<Real, SPT> q0, q1;
     // a frozen plane (multilevel mask, uniform over the workgroup) is frozen
     // for both phases: the whole workgroup leaves before the barrier
@@ -531,6 +530,7 @@ hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int bloc
 template <typename Real, int KIND>
 hipError_t launch_pair_spt(int spt, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
   if (spt == 2) return launch_pair_meas<Real, KIND, 2>(lat, a, nblocks, block, s);
+  if (spt == 3) return launch_pair_meas<Real, KIND, (sizeof(Real) == sizeof(float) ? 3 : 1)>(lat, a, nblocks, block, s);
   if (spt == 4) return launch_pair_meas<Real, KIND, (sizeof(Real) == sizeof(float) ? 4 : 1)>(lat, a, nblocks, block, s);
   return launch_pair_meas<Real, KIND, 1>(lat, a, nblocks, block, s);
 }
@@ -554,8 +554,9 @@ hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, i
                        hipStream_t s) {
   // single lattices only; whole wavefronts within the kernel's launch bound;
   // the plane decomposition of a.single must be one chunk per plane
-  const int maxb = prec == PREC_F64 ? 512 : 1024;
-  if (block < 64 || block > maxb || block % 64 != 0 || !(spt == 1 || spt == 2 || (spt == 4 && prec != PREC_F64)) ||
+  if (!(spt == 1 || spt == 2 || ((spt == 3 || spt == 4) && prec != PREC_F64))) return hipErrorInvalidValue;
+  const int maxb = pair_max_block(prec == PREC_F64 ? 8 : 4, spt);
+  if (block < 64 || block > maxb || block % 64 != 0 ||
       (a.meas != nullptr && (a.ml != 0 || a.reps != nullptr)))
     return hipErrorInvalidValue;
   // single lattice: one chunk per plane, one workgroup per plane (batched:

@@ -228,6 +228,35 @@ def test_fused_plaquette_fp32():
                 assert abs(p[k] - q[k]) <= 2e-6, (dims, s, p, q)
 
 
+@pytest.mark.parametrize("dims,nsw", [((16, 16, 16, 16), 20), ((48, 48, 48, 48), 10)])
+def test_fused_plaquette_fp32_vs_fp64(u1, orc, dims, nsw):
+    """The quantity the benchmark times: the FP32 sweep's in-sweep plaquette
+    (plaquette.cu:93-118 summed from the update's staple pieces) against the
+    FP64 plaquette of the same updated links widened to binary64, from the
+    GPU FP64 pass and from the oracle: <= 1e-6 relative for SS, ST and <P>."""
+    g = u1.geom(dims, 1.0)
+    lat = u1.Lattice.cold(g, u1.F32)
+    lat.hotstart(21)
+    say(f"fused fp32 plaquette vs fp64 {dims}")
+    worst = 0.0
+    for s in range(nsw):
+        p = lat.update_plaquette(1, 3, u1.BENCH_METRO, 21, s)
+        if s < nsw - 3:
+            continue
+        w = lat.astype(u1.F64)
+        q = w.plaquette()
+        if s == nsw - 1:
+            qo = orc.plaquette(orc.geom(dims, 1.0), w.to_host())
+            assert np.max(np.abs(q - qo)) <= 1e-12
+        for k in (0, 2):
+            worst = max(worst, abs(p[k] - q[k]) / abs(q[k]))
+        worst = max(worst, abs(u1.mean_plaquette(p) - u1.mean_plaquette(q)) / u1.mean_plaquette(q))
+        for k in (1, 3):
+            assert abs(p[k] - q[k]) <= 1e-6, (s, p, q)
+    print(f"[fused fp32 plaquette] {dims}: max relative error {worst:.2e}", flush=True)
+    assert worst <= 1e-6
+
+
 def test_stream_reference(u1):
     """u1_selftest_stream moves 36 n bytes at a plausible HBM rate and checks its arguments."""
     n = 1 << 21
@@ -324,6 +353,46 @@ def test_multilevel_fp64(u1, orc, mh_mode, n4, k4, n2, k2):
     assert np.allclose(f_gpu, f_orc, rtol=1e-11, atol=1e-16)
     # lattice passed in is not modified (multilevel works on a copy)
     assert np.array_equal(bits(lat.to_host()), bits(ref))
+
+
+def test_multilevel_eager_matches_graph(u1, orc):
+    """u1_set_graph_capture(0): the level-2 steps enqueued eagerly with the
+    same device counter give the graph replays' result bit for bit."""
+    dims = (4, 4, 4, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=23)
+    for s in range(3):
+        lat.update(1, 3, u1.BENCH_METRO, 23, s)
+    say("multilevel eager vs graph")
+    mh = u1.metro(1, 20, 0.5)
+    out = {}
+    try:
+        for graphs in (True, False):
+            u1.set_graph_capture(graphs)
+            out[graphs] = lat.multilevel(2, 1, 3, 2, 1, 2, u1.BENCH_METRO, 23, call_id=4, mh_mode=2, mh=mh,
+                                         field=True)
+    finally:
+        u1.set_graph_capture(True)
+    assert np.array_equal(bits(out[True][0].view(np.float64)), bits(out[False][0].view(np.float64)))
+    assert np.array_equal(bits(out[True][1].view(np.float64)), bits(out[False][1].view(np.float64)))
+    r_orc = orc.multilevel(og, lat.to_host(), 2, 1, 3, 2, 1, 2, orc.metro(1, 4, 0.5), 23, call_id=4, mh_mode=2,
+                           mh=orc.metro(1, 20, 0.5))
+    assert np.allclose(out[False][0], r_orc, rtol=1e-11, atol=1e-16)
+
+
+@pytest.mark.parametrize("n4,n2", [(3, 4), (5, 2)])
+def test_multilevel_level_averages_fp64(u1, orc, n4, n2):
+    """k4 = k2 = 0: the captured level-2 graph replays n4 * n2 times over one
+    frozen field, so the level averages (1/n2, 1/n4, T2 cleared per level-4
+    step; multilevel.cu:665-700) must give back Poly2 with the multihit links."""
+    dims = (4, 4, 4, 8)
+    g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=44)
+    for s in range(3):
+        lat.update(1, 3, u1.BENCH_METRO, 44, s)
+        orc.update(og, ref, 1, 3, orc.metro(1, 4, 0.5), 44, s)
+    say(f"multilevel level averages n4={n4} n2={n2}")
+    r_gpu = lat.multilevel(n4, 0, n2, 0, 1, 3, u1.BENCH_METRO, 44, mh_mode=1)
+    assert np.allclose(r_gpu, orc.poly2(og, ref, 1), rtol=1e-11, atol=1e-16)
+    assert np.allclose(r_gpu, lat.poly2(u1.POLY_MULTIHIT), rtol=1e-11, atol=1e-16)
 
 
 def test_multilevel_errors(u1):

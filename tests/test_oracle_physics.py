@@ -255,6 +255,71 @@ def test_multilevel_reduces_to_poly2_multihit(orc):
     assert np.allclose(ml, p2, rtol=1e-12, atol=1e-15)
 
 
+@pytest.mark.parametrize("n4,n2", [(3, 4), (5, 2)])
+def test_multilevel_level_averages_of_frozen_field(orc, n4, n2):
+    """With no sublattice updates (k4 = k2 = 0) every level-2 measurement sees
+    the same field, so the 1/n2 and 1/n4 averages (multilevel.cu:665, 668) and
+    the T2 clear before each level-4 step (multilevel.cu:674) must give back
+    Poly2 with the multihit links.  A missing norm or reset scales C(R) by a
+    power of n2 or n4."""
+    dims = (4, 4, 4, 8)
+    g = orc.geom(dims, beta=1.0)
+    lat = orc.hotstart(g, 44)
+    for s in range(3):
+        orc.update(g, lat, 1, 3, orc.metro(1, 4, 0.5), 44, s)
+    ml = orc.multilevel(g, lat, n4, 0, n2, 0, 1, 3, orc.metro(1, 4, 0.5), 44, mh_mode=1)
+    p2 = orc.poly2(g, lat, 1)
+    assert np.allclose(ml, p2, rtol=1e-12, atol=1e-16)
+
+
+def _multilevel_numpy(orc, g, lat, n4, k4, n2, k2, metrop, ovrn, m, seed, call_id, mh_mode, mh):
+    """The two-level algorithm (multilevel.cu:639-700, T2 kernel :331-360, T2 -> T4
+    :439-456, final :532-553) recomposed in numpy from the oracle's pinned
+    primitives: masked updates (Philox streams 3 and 4, sweep index = step
+    counter, c3 = call_id) and the temporal-link field (stream 5, c1 = level-2
+    step).  Axis order of the lexicographic field: (t, z, y, x)."""
+    Nx, Ny, Nz, Nt = g.dims[0], g.dims[1], g.dims[2], g.dims[3]
+    R = Nx // 2
+    lat = lat.copy()
+    T4 = np.zeros((Nt // 4, 3, R, Nz, Ny, Nx), complex)
+    for i in range(n4):
+        for j in range(k4):
+            orc.update(g, lat, metrop, ovrn, m, seed, i * k4 + j, stream=3, c3=call_id, mask_ml=4)
+        T2 = np.zeros((Nt // 2, 3, R, Nz, Ny, Nx), complex)
+        for k in range(n2):
+            for l in range(k2):
+                orc.update(g, lat, metrop, ovrn, m, seed, (i * n2 + k) * k2 + l, stream=4, c3=call_id, mask_ml=2)
+            tl = orc.temporal_links(g, lat, mh_mode, mh, seed, 5, i * n2 + k, call_id).reshape(Nt, Nz, Ny, Nx)
+            two = tl[0::2] * tl[1::2]  # line of two temporal links from t = 2l
+            for d, ax in enumerate((3, 2, 1)):  # x, y, z in the (l, z, y, x) array
+                for r in range(1, R + 1):
+                    T2[:, d, r - 1] += two * np.conj(np.roll(two, -r, axis=ax))
+        T2 /= n2
+        T4 += T2[0::2] * T2[1::2]
+    T4 /= n4
+    return np.prod(T4, axis=0).mean(axis=(0, 2, 3, 4))
+
+
+@pytest.mark.parametrize("mh_mode", [1, 2])
+def test_multilevel_matches_numpy_recomposition(orc, mh_mode):
+    """Level averages over fields that do change (k4, k2 > 0, n4 = 2, n2 = 3),
+    against the numpy recomposition above: pins the T2/T4 products, the level
+    norms and the per-level reset where the measured values differ."""
+    dims = (4, 4, 4, 8)
+    g = orc.geom(dims, beta=1.0)
+    lat = orc.hotstart(g, 71)
+    m = orc.metro(1, 4, 0.5)
+    mh = orc.metro(1, 10, 0.5)
+    for s in range(3):
+        orc.update(g, lat, 1, 3, m, 71, s)
+    ml = orc.multilevel(g, lat, 2, 1, 3, 1, 1, 2, m, 71, call_id=5, mh_mode=mh_mode, mh=mh)
+    ref = _multilevel_numpy(orc, g, lat, 2, 1, 3, 1, 1, 2, m, 71, 5, mh_mode, mh)
+    assert np.allclose(ml, ref, rtol=1e-11, atol=1e-15)
+    # the level averaging matters here: a single level-2 measurement differs
+    one = _multilevel_numpy(orc, g, lat, 1, 1, 1, 1, 1, 2, m, 71, 5, mh_mode, mh)
+    assert not np.allclose(ml, one, rtol=1e-3)
+
+
 def test_multilevel_masks_freeze_boundaries(orc):
     dims = (4, 4, 4, 8)
     g = orc.geom(dims)
