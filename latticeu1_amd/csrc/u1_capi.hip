@@ -370,10 +370,11 @@ int time_pair(int prec, int kind, void* lat, const PhaseArgs& a, const Cfg& c, f
   hipEvent_t e0, e1;
   HIPCHK(es.create(&e0));
   HIPCHK(es.create(&e1));
-  HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, G.st));  // warm-up
+  const int plane = a.single.nxh * a.single.N[1];
+  HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, plane, G.st));  // warm-up
   HIPCHK(hipEventRecord(e0, G.st));
   for (int rep = 0; rep < 4; ++rep)
-    HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, G.st));
+    HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, c.block, c.spt, plane, G.st));
   HIPCHK(hipEventRecord(e1, G.st));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
@@ -388,6 +389,8 @@ int pair_cfg(int prec, int kind, const u1_geom* g, const void* lat, const PhaseA
              Cfg* cfg) {
   *cfg = Cfg{0, 1, 0, 0};
   if (G.pair == 0) return U1_OK;
+  if (proto.single.nxh * proto.single.N[1] > u1d::pair_max_plane((int)elem_size(prec)))
+    return U1_OK;  // the plane's angles do not fit the pair kernel's LDS: two launches
   const char* name = kind == u1d::KIND_METROPOLIS ? "MetropolisPair" : "OverRelaxationPair";
   const std::string key = tune_key(name, g, prec);
   const std::vector<Cfg> cands = pair_candidates(prec, proto.single);
@@ -538,7 +541,7 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
         if (pair) {
           a.single = rp;
           a.xcd = (uint32_t)pc.xcd;
-          HIPCHK(u1d::launch_pair(prec, kind, lat, a, rp.nblk, pc.block, pc.spt, G.st));
+          HIPCHK(u1d::launch_pair(prec, kind, lat, a, rp.nblk, pc.block, pc.spt, rp.nxh * rp.N[1], G.st));
           ++mu;
         } else {
           a.single = r;
@@ -630,6 +633,7 @@ struct u1_ensemble {
   RepDesc* pair_dev = nullptr;
   uint16_t* pair_map = nullptr;
   int pair_blocks = 0;
+  int pair_plane = 0;  // largest plane (Nx/2 * Ny sites) of the replicas: the pair kernel's LDS
 };
 
 namespace {
@@ -677,6 +681,9 @@ int ensemble_settle(u1_ensemble* e) {
 int ensemble_build_pair(u1_ensemble* e, Cfg c) {
   e->pcfg = Cfg{0, 1, 0, 0};  // set to c only once its tables are in place
   if (c.block == 0) return U1_OK;
+  int plane = 0;
+  for (const RepDesc& r : e->upd_host) plane = std::max(plane, r.nxh * r.N[1]);
+  if (plane > u1d::pair_max_plane((int)elem_size(e->prec))) return U1_OK;  // planes too large for the LDS exchange
   std::vector<RepDesc> reps(e->upd_host);
   std::vector<uint16_t> pmap;
   int nb = 0;
@@ -696,6 +703,7 @@ int ensemble_build_pair(u1_ensemble* e, Cfg c) {
   HIPCHK(hipMemcpyAsync(e->pair_dev, reps.data(), e->nrep * sizeof(RepDesc), hipMemcpyHostToDevice, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
   e->pair_blocks = nb;
+  e->pair_plane = plane;
   e->pcfg = c;
   return U1_OK;
 }
@@ -734,7 +742,8 @@ int ensemble_enqueue_phases(u1_ensemble* e, void* lat, double* mrows = nullptr) 
           b.blockmap = e->pair_map;
           b.meas = nullptr;
           b.xcd = (uint32_t)e->pcfg.xcd;
-          HIPCHK(u1d::launch_pair(e->prec, kind, lat, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt, G.st));
+          HIPCHK(u1d::launch_pair(e->prec, kind, lat, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt, e->pair_plane,
+                                  G.st));
           ++mu;
           continue;
         }
@@ -953,7 +962,8 @@ int time_ensemble_classes(u1_ensemble* e, double* ms, double* per_sweep) {
       b.parity = ph[(size_t)i % ph.size()].first;
       b.pass = (uint32_t)(i / (int)ph.size());
       if (pair)
-        HIPCHK(u1d::launch_pair(e->prec, kind, scratch.p, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt, G.st));
+        HIPCHK(u1d::launch_pair(e->prec, kind, scratch.p, b, e->pair_blocks, e->pcfg.block, e->pcfg.spt,
+                                e->pair_plane, G.st));
       else
         HIPCHK(u1d::launch_phase(e->prec, kind, ph[(size_t)i % ph.size()].second, scratch.p, b, e->upd_blocks,
                                  e->cfg.block, e->cfg.spt, G.st));
@@ -1827,7 +1837,9 @@ int time_launch_classes(const u1_geom* g, int prec, const void* lat, int metrop,
       b.parity = ph[(size_t)i % ph.size()].first;
       b.pass = (uint32_t)(i / (int)ph.size());
       b.meas = cmeas ? G.partials : nullptr;
-      if (pair) HIPCHK(u1d::launch_pair(prec, kind, scratch.p, b, b.single.nblk, pc.block, pc.spt, G.st));
+      if (pair)
+        HIPCHK(u1d::launch_pair(prec, kind, scratch.p, b, b.single.nblk, pc.block, pc.spt, b.single.nxh * b.single.N[1],
+                                G.st));
       else
         HIPCHK(u1d::launch_phase(prec, kind, ph[(size_t)i % ph.size()].second, scratch.p, b, b.single.nblk,
                                  cfg[kind].block, cfg[kind].spt, G.st));

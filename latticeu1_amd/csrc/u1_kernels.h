@@ -40,15 +40,18 @@ hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseA
 
 // The phases (parity, 0) and (parity, 1) in one launch (k_pair): one
 // workgroup per (z, t) plane.  A single lattice: a.single carries
-// set_phase_blocks(r, Nx/2 * Ny) and nblocks = Nz * Nt.  Batched replicas
-// (a.reps, a.blockmap): every replica descriptor carries its own
-// set_phase_blocks(r, Nx/2 * Ny) with blk_begin its first workgroup, and the
-// block map covers nblocks; no measurement.  spt 1, 2 (both precisions), 3 or
-// 4 (binary32); block a multiple of 64 up to pair_max_block.  a.meas makes
-// phase (parity, 1) measuring.
+// set_phase_blocks(r, Nx/2 * Ny), nblocks = Nz * Nt and plane_sites =
+// Nx/2 * Ny.  Batched replicas (a.reps, a.blockmap): every replica descriptor
+// carries its own set_phase_blocks(r, Nx/2 * Ny) with blk_begin its first
+// workgroup, the block map covers nblocks, plane_sites is the largest plane;
+// no measurement.  spt 1, 2 (both precisions), 3 or 4 (binary32); block a
+// multiple of 64 up to pair_max_block; plane_sites up to pair_max_plane (the
+// plane's new theta_0 is exchanged through LDS).  a.meas makes phase
+// (parity, 1) measuring.
 constexpr int pair_max_block(int elem_bytes, int spt) { return elem_bytes == 8 || spt >= 3 ? 512 : 1024; }
+constexpr int pair_max_plane(int elem_bytes) { return 65536 / elem_bytes; }
 hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
-                       hipStream_t st);
+                       int plane_sites, hipStream_t st);
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t st);
 

@@ -65,7 +65,7 @@ struct StapleIn {
 };
 
 // Loads the staple angles.  With DEFER0 the two of them that read theta_0 at
-// the link's own parity (c and d of nu = 0) are left to staple_load_nu0.
+// the link's own parity (c and d of nu = 0) are left to phase_load_nu0_lds.
 template <typename Real, int MU, bool DEFER0 = false>
 __device__ __forceinline__ void staple_load(const Real* __restrict__ lat, const RepDesc& r, uint32_t same,
                                             uint32_t opp, const int32_t (&fwd)[4], const int32_t (&bwd)[4],
@@ -86,15 +86,6 @@ __device__ __forceinline__ void staple_load(const Real* __restrict__ lat, const 
     in.f[j] = ld(pn, opp + bwd[nu]);
   }
 }
-template <typename Real, int MU>
-__device__ __forceinline__ void staple_load_nu0(const Real* __restrict__ lat, uint32_t same,
-                                                const int32_t (&fwd)[4], const int32_t (&bwd)[4],
-                                                StapleIn<Real>& in) {
-  static_assert(MU != 0, "nu = 0 is a staple direction only for MU != 0");
-  in.c[0] = ld(lat, same);
-  in.d[0] = ld(lat, same + fwd[MU] + bwd[0]);
-}
-
 // Staple of link (x, MU) (staple.h:79-102) from its angles, with the operation
 // order of the oracle.  With MEAS the plaquette pieces are collected on the
 // side (the staple is unchanged).
@@ -253,18 +244,26 @@ __device__ __forceinline__ void phase_load(const Real* __restrict__ lat_base, co
   }
 }
 
-// The loads phase_load<..., DEFER0 = true> left out.
-template <typename Real, int MU, int SPT>
-__device__ __forceinline__ void phase_load_deferred(const Real* __restrict__ lat_base, const RepDesc& r,
-                                                    PhaseIn<Real, SPT>& q) {
+// The two angles phase_load<..., DEFER0 = true> left out -- theta_0 at the
+// link's own parity at x and x + e1 - e0, both in the site's (z, t) plane --
+// from lds_th, where phase 0 of this workgroup left the plane's new theta_0
+// by in-plane index (id - pbase).
+template <typename Real, int SPT>
+__device__ __forceinline__ void phase_load_nu0_lds(const Real* lds_th, uint32_t pbase, PhaseIn<Real, SPT>& q) {
 #pragma unroll
-  for (int k = 0; k < SPT; ++k)
-    staple_load_nu0<Real, MU>(lat_base + r.lat_off, q.same[k], q.ps[k].fwd, q.ps[k].bwd, q.in[k]);
+  for (int k = 0; k < SPT; ++k) {
+    const uint32_t i = q.ps[k].id - pbase;
+    q.in[k].c[0] = lds_th[i];
+    q.in[k].d[0] = lds_th[i + (uint32_t)(q.ps[k].fwd[1] + q.ps[k].bwd[0])];
+  }
 }
 
+// lds_th (nullable): the new angles are also written to lds_th[id - pbase],
+// the site's index within its (z, t) plane (fused pairs, phase 0).
 template <typename Real, int MU, int KIND, int SPT, bool MEAS>
 __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
-                                              PhaseIn<Real, SPT>& q, double (&v)[2], uint32_t& acc_wave) {
+                                              PhaseIn<Real, SPT>& q, double (&v)[2], uint32_t& acc_wave,
+                                              Real* lds_th = nullptr, uint32_t pbase = 0) {
   Real* pmu = lat_base + r.lat_off + (size_t)MU * r.V;
   const bool aniso = r.aniso != 1.0;
   const Real xi = sizeof(Real) == sizeof(float) ? (Real)r.aniso_f : (Real)r.aniso;
@@ -340,7 +339,10 @@ __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const
   }
 #pragma unroll
   for (int k = 0; k < SPT; ++k)
-    if (ps[k].valid) st(pmu, same[k], th[k]);
+    if (ps[k].valid) {
+      st(pmu, same[k], th[k]);
+      if (lds_th) lds_th[ps[k].id - pbase] = th[k];
+    }
   if (MEAS) {
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
@@ -402,18 +404,22 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 // race free and gives the results of the two separate launches bit for bit:
 //  * the staple of (x, 1) reads the phase-0 links theta_0 at this parity only
 //    at x and x + e1 - e0, both in the same (z, t) plane, so in this
-//    workgroup, and written before the barrier;
+//    workgroup: phase 0 also leaves them in LDS (P angles of the plane, by
+//    in-plane index), and phase 1 reads those two from there after the
+//    barrier instead of reloading them;
 //  * phase (parity, 0) reads theta_1 at this parity only at x and x + e0 - e1,
-//    also in the same plane, so before this workgroup's phase 1 rewrites them;
+//    also in the same plane, so before this workgroup's phase 1 rewrites them
+//    (its loads have returned before the barrier);
 //  * nothing else either phase reads is written by the launch.
-// __syncthreads orders the workgroup's global stores before its later loads.
 // The pair reads the lattice once (4V angles) and writes V/2 + V/2: 5V angles
-// against 9V for two launches.  MEAS applies to phase (parity, 1).
-// When one step covers the plane (P <= SPT * blockDim.x, SPT <= 3), every
-// load of both phases except the two phase-0 results is issued before phase 0
-// computes, so phase 1 waits on the barrier and two cache-hit loads only.
+// against 9V for two launches.  MEAS applies to phase (parity, 1).  Dynamic
+// LDS: P angles (pair_lds_bytes).  When one step covers the plane (P <= SPT *
+// blockDim.x, SPT <= 3), every load of phase 1 is issued before phase 0
+// computes, so phase 1 waits on the barrier and two LDS reads only.
 template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
 __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real* __restrict__ lat_base, PhaseArgs a) {
+  extern __shared__ __align__(8) unsigned char pair_lds[];
+  Real* th0 = reinterpret_cast<Real*>(pair_lds);
   const uint32_t L = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const RepDesc& r = BATCHED ? a.reps[a.blockmap[L]] : a.single;
   const uint32_t b = L - (uint32_t)r.blk_begin;  // the replica's plane (one chunk per plane)
@@ -428,26 +434,30 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
 #pragma unroll
     for (int k = 0; k < SPT; ++k) phase_site(r, b, threadIdx.x + k * blockDim.x, step, a.parity, q0.ps[k]);
     if (a.ml != 0 && (q0.ps[0].t & (uint32_t)(a.ml - 1)) == 0) return;
+    const uint32_t pbase = ((uint32_t)q0.ps[0].z + (uint32_t)r.N[2] * q0.ps[0].t) * P;  // the plane's first id
 #pragma unroll
     for (int k = 0; k < SPT; ++k) q1.ps[k] = q0.ps[k];  // the same sites
     phase_load<Real, 0, SPT>(lat_base, a, r, q0);
     phase_load<Real, 1, SPT, true>(lat_base, a, r, q1);
-    phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q0, v0, acc0);
+    phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q0, v0, acc0, th0, pbase);
     __syncthreads();
-    phase_load_deferred<Real, 1, SPT>(lat_base, r, q1);
+    phase_load_nu0_lds<Real, SPT>(th0, pbase, q1);
     phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q1, v, acc1);
   } else {
+    uint32_t pbase = 0;
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
       if (phase_sites<0, SPT>(a, r, b, base + threadIdx.x, step, q.ps)) return;  // frozen plane
+      pbase = ((uint32_t)q.ps[0].z + (uint32_t)r.N[2] * q.ps[0].t) * P;
       phase_load<Real, 0, SPT>(lat_base, a, r, q);
-      phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q, v0, acc0);
+      phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q, v0, acc0, th0, pbase);
     }
     __syncthreads();
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
       phase_sites<1, SPT>(a, r, b, base + threadIdx.x, step, q.ps);
-      phase_load<Real, 1, SPT>(lat_base, a, r, q);
+      phase_load<Real, 1, SPT, true>(lat_base, a, r, q);
+      phase_load_nu0_lds<Real, SPT>(th0, pbase, q);
       phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
     }
   }
@@ -517,29 +527,31 @@ PhaseArgs with_f32_constants(const PhaseArgs& a) {
 }
 
 template <typename Real, int KIND, int SPT>
-hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
+hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int block, size_t lds, hipStream_t s) {
   if (a.reps != nullptr)  // batched replicas: no measuring pairs
-    k_pair<Real, KIND, SPT, false, true><<<nblocks, block, 0, s>>>(lat, a);
+    k_pair<Real, KIND, SPT, false, true><<<nblocks, block, lds, s>>>(lat, a);
   else if (a.meas != nullptr)
-    k_pair<Real, KIND, SPT, true><<<nblocks, block, 0, s>>>(lat, a);
+    k_pair<Real, KIND, SPT, true><<<nblocks, block, lds, s>>>(lat, a);
   else
-    k_pair<Real, KIND, SPT, false><<<nblocks, block, 0, s>>>(lat, a);
+    k_pair<Real, KIND, SPT, false><<<nblocks, block, lds, s>>>(lat, a);
   return hipGetLastError();
 }
 
 template <typename Real, int KIND>
-hipError_t launch_pair_spt(int spt, Real* lat, const PhaseArgs& a, int nblocks, int block, hipStream_t s) {
-  if (spt == 2) return launch_pair_meas<Real, KIND, 2>(lat, a, nblocks, block, s);
-  if (spt == 3) return launch_pair_meas<Real, KIND, (sizeof(Real) == sizeof(float) ? 3 : 1)>(lat, a, nblocks, block, s);
-  if (spt == 4) return launch_pair_meas<Real, KIND, (sizeof(Real) == sizeof(float) ? 4 : 1)>(lat, a, nblocks, block, s);
-  return launch_pair_meas<Real, KIND, 1>(lat, a, nblocks, block, s);
+hipError_t launch_pair_spt(int spt, Real* lat, const PhaseArgs& a, int nblocks, int block, size_t lds,
+                           hipStream_t s) {
+  constexpr bool F = sizeof(Real) == sizeof(float);
+  if (spt == 2) return launch_pair_meas<Real, KIND, 2>(lat, a, nblocks, block, lds, s);
+  if (spt == 3) return launch_pair_meas<Real, KIND, (F ? 3 : 1)>(lat, a, nblocks, block, lds, s);
+  if (spt == 4) return launch_pair_meas<Real, KIND, (F ? 4 : 1)>(lat, a, nblocks, block, lds, s);
+  return launch_pair_meas<Real, KIND, 1>(lat, a, nblocks, block, lds, s);
 }
 
 template <typename Real>
-hipError_t launch_pair_prec(int kind, Real* lat, const PhaseArgs& a, int nblocks, int block, int spt,
+hipError_t launch_pair_prec(int kind, Real* lat, const PhaseArgs& a, int nblocks, int block, int spt, size_t lds,
                             hipStream_t s) {
-  if (kind == KIND_METROPOLIS) return launch_pair_spt<Real, KIND_METROPOLIS>(spt, lat, a, nblocks, block, s);
-  return launch_pair_spt<Real, KIND_OVERRELAX>(spt, lat, a, nblocks, block, s);
+  if (kind == KIND_METROPOLIS) return launch_pair_spt<Real, KIND_METROPOLIS>(spt, lat, a, nblocks, block, lds, s);
+  return launch_pair_spt<Real, KIND_OVERRELAX>(spt, lat, a, nblocks, block, lds, s);
 }
 
 }  // namespace
@@ -551,20 +563,23 @@ hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseA
 }
 
 hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
-                       hipStream_t s) {
-  // single lattices only; whole wavefronts within the kernel's launch bound;
-  // the plane decomposition of a.single must be one chunk per plane
+                       int plane_sites, hipStream_t s) {
+  // whole wavefronts within the kernel's launch bound; the plane's angles in LDS
   if (!(spt == 1 || spt == 2 || ((spt == 3 || spt == 4) && prec != PREC_F64))) return hipErrorInvalidValue;
-  const int maxb = pair_max_block(prec == PREC_F64 ? 8 : 4, spt);
-  if (block < 64 || block > maxb || block % 64 != 0 ||
+  const int elem = prec == PREC_F64 ? 8 : 4;
+  const int maxb = pair_max_block(elem, spt);
+  if (block < 64 || block > maxb || block % 64 != 0 || plane_sites < 1 || plane_sites > pair_max_plane(elem) ||
       (a.meas != nullptr && (a.ml != 0 || a.reps != nullptr)))
     return hipErrorInvalidValue;
-  // single lattice: one chunk per plane, one workgroup per plane (batched:
-  // the replica table and block map carry the same layout per replica)
-  if (a.reps == nullptr && (a.single.bpp != 1 || nblocks != a.single.N[2] * a.single.N[3]))
+  // single lattice: one chunk per plane, one workgroup per plane, plane_sites
+  // its plane (batched: the replica table and block map carry the same layout
+  // per replica, plane_sites the largest plane)
+  if (a.reps == nullptr && (a.single.bpp != 1 || nblocks != a.single.N[2] * a.single.N[3] ||
+                            plane_sites != a.single.nxh * a.single.N[1]))
     return hipErrorInvalidValue;
-  if (prec == PREC_F64) return launch_pair_prec<double>(kind, (double*)lat_base, a, nblocks, block, spt, s);
-  return launch_pair_prec<float>(kind, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, s);
+  const size_t lds = (size_t)plane_sites * (size_t)elem;
+  if (prec == PREC_F64) return launch_pair_prec<double>(kind, (double*)lat_base, a, nblocks, block, spt, lds, s);
+  return launch_pair_prec<float>(kind, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, lds, s);
 }
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t s) {

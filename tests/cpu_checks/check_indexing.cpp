@@ -138,7 +138,10 @@ int main() {
     }
   }
   // fused-pair decomposition: one chunk (workgroup) per plane, the plane
-  // covered in steps of spt * block sites (tail threads in bounds)
+  // covered in steps of spt * block sites (tail threads in bounds); the LDS
+  // exchange of phase 0's theta_0: in-plane index id - pbase and that of
+  // x + e1 - e0 (id + fwd[1] + bwd[0]) inside [0, P) for every thread, equal
+  // to the brute-force in-plane position of the neighbour for valid sites
   long pairsites = 0;
   for (const auto& G : geoms) {
     u1d::RepDesc r{};
@@ -157,24 +160,40 @@ int main() {
         u1d::set_phase_tiles(r, tiled != 0);
         if (tiled && !r.tile) continue;
         if (r.bpp != 1 || r.nblk != G[2] * G[3]) { printf("pair: not one chunk per plane\n"); return 1; }
-        std::vector<unsigned char> seen(r.HV, 0);
-        for (int blk = 0; blk < r.nblk; ++blk) {
-          uint32_t t0 = 0;
-          for (uint32_t base = 0; base < P; base += (uint32_t)step)
-            for (uint32_t tid = 0; tid < (uint32_t)step; ++tid) {
-              u1d::PhaseSite ps;
-              u1d::phase_site(r, blk, base + tid, (uint32_t)step, 0, ps);
-              if (ps.id >= (uint32_t)r.HV) { printf("pair: id out of range\n"); return 1; }
-              if (base == 0 && tid == 0) t0 = ps.t;
-              if (ps.t != t0) { printf("pair: t not uniform over the workgroup\n"); return 1; }
-              if (!ps.valid) continue;
-              if (seen[ps.id]) { printf("pair: duplicate site\n"); return 1; }
-              seen[ps.id] = 1;
-              ++pairsites;
-            }
+        for (int parity = 0; parity < 2; ++parity) {
+          std::vector<unsigned char> seen(r.HV, 0);
+          for (int blk = 0; blk < r.nblk; ++blk) {
+            uint32_t t0 = 0;
+            int32_t z0 = 0;
+            for (uint32_t base = 0; base < P; base += (uint32_t)step)
+              for (uint32_t tid = 0; tid < (uint32_t)step; ++tid) {
+                u1d::PhaseSite ps;
+                u1d::phase_site(r, blk, base + tid, (uint32_t)step, parity, ps);
+                if (ps.id >= (uint32_t)r.HV) { printf("pair: id out of range\n"); return 1; }
+                if (base == 0 && tid == 0) {
+                  t0 = ps.t;
+                  z0 = ps.z;
+                }
+                if (ps.t != t0 || ps.z != z0) { printf("pair: z/t not uniform over the workgroup\n"); return 1; }
+                const uint32_t pbase = ((uint32_t)ps.z + (uint32_t)G[2] * ps.t) * P;
+                const uint32_t i = ps.id - pbase, j = i + (uint32_t)(ps.fwd[1] + ps.bwd[0]);
+                if (i >= P || j >= P) { printf("pair: LDS index out of the plane\n"); return 1; }
+                if (!ps.valid) continue;
+                if (seen[ps.id]) { printf("pair: duplicate site\n"); return 1; }
+                seen[ps.id] = 1;
+                // x + e1 - e0 from explicit coordinates, as an in-plane index
+                int x[4];
+                const int za = ps.id / (G[0] / 2), zb = za / G[1];
+                x[1] = za - zb * G[1]; x[3] = zb / G[2]; x[2] = zb - x[3] * G[2];
+                x[0] = 2 * (int)ps.id + ((x[1] + x[2] + x[3] + parity) & 1) - za * G[0];
+                int y[4] = {(x[0] - 1 + G[0]) % G[0], (x[1] + 1) % G[1], x[2], x[3]};
+                if ((uint32_t)(lexi(G, y) >> 1) - pbase != j) { printf("pair: LDS neighbour wrong\n"); return 1; }
+                ++pairsites;
+              }
+          }
+          for (int k = 0; k < r.HV; ++k)
+            if (!seen[k]) { printf("pair: site %d not covered\n", k); return 1; }
         }
-        for (int k = 0; k < r.HV; ++k)
-          if (!seen[k]) { printf("pair: site %d not covered\n", k); return 1; }
       }
   }
   printf("pair decomposition ok (%ld site visits)\n", pairsites);

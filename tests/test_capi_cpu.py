@@ -59,7 +59,8 @@ def test_tune_cache_roundtrip(tmp_path):
                  "8x8x8x8\tMetropolis_f64\tthreads=halfvolume\t128\t1\t1\t0\t1\t1\t0\t# x\n"
                  "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t0\t1\t1\t0\t# x\n"
                  "8x8x8x8\tOverRelaxationPair_f32\tthreads=halfvolume\t576\t2\t2\t0\t1\t1\t0\t# x\n"
-                 "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t0\t1\t1\t0\t# x\n")
+                 "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t0\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tMetropolisPair_f32\tthreads=halfvolume\t384\t3\t2\t0\t1\t1\t0\t# x\n")
     u1.tune_load(str(p))
     n = C.c_int32(0)
     assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 4
@@ -73,6 +74,8 @@ def test_tune_cache_roundtrip(tmp_path):
     # fused pair shapes (workgroup = whole plane, up to 1024 threads) and "two launches" (block 0)
     assert "8x8x8x8\tOverRelaxationPair_f32\tthreads=halfvolume\t576\t2\t2\t" in txt
     assert "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t" in txt
+    # three sites per thread (binary32 pairs, up to 512 threads)
+    assert "8x8x8x8\tMetropolisPair_f32\tthreads=halfvolume\t384\t3\t2\t" in txt
     bad = tmp_path / "bad.csv"
     bad.write_text("not a cache\n")
     with pytest.raises(u1.U1Error):

@@ -53,28 +53,37 @@ def test_multihit_24x3x48_fp64_parity_fixed_config(u1, orc):
     c_gpu = lat.poly2(2, mh, seed=33, c1=7)
     c_orc = orc.poly2(og, ref, 2, orc.metro(1, 100, 0.5), seed=33, c1=7)
     assert np.allclose(c_gpu, c_orc, rtol=1e-11, atol=1e-18)
-    # FP32 links on the same configuration.  Two errors are stated:
-    #  * storage: the FP64 configuration rounded to binary32 angles, measured
-    #    by the FP32 kernels, against the oracle on the FP64 angles (the
-    #    rounding of the angles grows with the Nt = 48 factors of each loop);
-    #  * kernel: the same FP32 configuration, widened back to binary64 and
-    #    measured by the oracle, against the FP32 kernels' measurement.
-    # Analytic multihit is deterministic; the stochastic estimator is a
-    # different draw in FP32 (accept decisions on near-ties differ between
-    # precisions), so it is compared against its own statistical scale.
+    # FP32 links on a fixed thermalised configuration (200 more FP64 sweeps on
+    # the GPU, whose FP64 updates were just shown identical to the oracle's).
+    # On one configuration C(R) is a sum of 3 SV terms P(x) P*(x + R) whose
+    # phases nearly cancel, so its error is stated against the scale of its
+    # terms, S(R) = (1/3SV) sum |P(x)| |P(x + R)| (computed from the oracle's
+    # P(x)): |FP32 - FP64| / S(R) <= 1e-6 for R = 1..12, for the storage error
+    # (the FP64 configuration rounded to binary32 angles, measured by the FP32
+    # kernels, against the oracle on the FP64 angles) and the kernel error (the
+    # oracle on the same binary32 angles widened to binary64).  Analytic
+    # multihit is deterministic; the stochastic estimator is a different draw
+    # in FP32 (accept decisions on near-ties differ between precisions), so it
+    # is compared against its own statistical scale.
+    for s in range(3, 203):
+        lat.update(1, 3, u1.BENCH_METRO, 33, s)
+    ref = lat.to_host()
     a64 = orc.poly2(og, ref, 1)
+    P = orc.poly_volume(og, ref, 1).reshape(dims[2], dims[1], dims[0])
+    scale = np.array([np.mean([np.mean(np.abs(P) * np.abs(np.roll(P, -r, axis=ax))) for ax in (2, 1, 0)])
+                      for r in range(1, 13)])
     f32 = lat.astype(u1.F32)
     a32 = f32.poly2(1)
     a32_64 = orc.poly2(og, f32.astype(u1.F64).to_host(), 1)
-    scale = np.abs(a64[0])  # C(1), the largest value of the correlator
     rel_storage = np.abs(a32 - a64) / scale
     rel_kernel = np.abs(a32 - a32_64) / scale
+    c_orc = orc.poly2(og, ref, 2, orc.metro(1, 100, 0.5), seed=33, c1=7)
     c32 = f32.poly2(2, mh, seed=33, c1=7)
     rel_s = np.max(np.abs(c32 - c_orc)) / np.max(np.abs(c_orc))
-    say(f"multihit 24^3x48 fixed config, analytic multihit correlator R=1..12, |FP32 - FP64| / C(1): storage "
-        f"max {rel_storage.max():.2e} ({np.array2string(rel_storage, precision=1)}), kernel max "
-        f"{rel_kernel.max():.2e}; per-R relative {np.array2string(np.abs(a32 - a64) / np.abs(a64), precision=1)}; "
-        f"stochastic (100 hits) {rel_s:.2e}")
+    say(f"multihit 24^3x48 fixed config, analytic multihit correlator R=1..12, |FP32 - FP64| / S(R): storage "
+        f"max {rel_storage.max():.2e}, kernel max {rel_kernel.max():.2e}; |C(R)| / S(R) = "
+        f"{np.array2string(np.abs(a64) / scale, precision=1)}; per-R relative "
+        f"{np.array2string(np.abs(a32 - a64) / np.abs(a64), precision=1)}; stochastic (100 hits) {rel_s:.2e}")
     assert len(a64) == 12
     assert rel_storage.max() <= 1e-6
     assert rel_kernel.max() <= 1e-6
@@ -140,32 +149,35 @@ def test_multilevel_24x4_fp32_fixed_config(u1, orc):
     """The multilevel correlator (multilevel.cu:532-553) of one fixed,
     thermalised 24^4 configuration with analytic multihit and no sublattice
     updates (k4 = k2 = 0; the level-2 graph replays n4 * n2 = 12 times): FP32
-    kernels on the binary32-rounded angles against the FP64 kernels and the
-    oracle on the binary64 angles, <= 1e-6 relative to C(1) for R = 1..10
-    (storage rounding included), and against the oracle on the same binary32
-    angles widened to binary64 (kernel error alone)."""
+    kernels on the binary32-rounded angles against the oracle on the binary64
+    angles (storage rounding included) and on the same binary32 angles
+    widened to binary64 (kernel error alone), R = 1..10.  On one
+    configuration C(R) is a sum of 3 SV products whose phases nearly cancel,
+    so the bar is against the scale of its terms, S(R) = mean |term| from the
+    oracle's per-site field (MultiLevelField): |FP32 - FP64| / S(R) <= 1e-6."""
     dims = (24, 24, 24, 24)
     g = u1.geom(dims, 1.0)
     og = orc.geom(dims, 1.0)
     lat = u1.Lattice.cold(g, u1.F64)
     lat.hotstart(404)
-    for s in range(50):
+    for s in range(200):
         lat.update(1, 3, u1.BENCH_METRO, 404, s)
     say("multilevel fp32 vs fp64 fixed configuration: 24^4, n4=3 k4=0 n2=4 k2=0, analytic multihit")
     r64 = lat.multilevel(3, 0, 4, 0, 1, 3, u1.BENCH_METRO, 404, call_id=3, mh_mode=1)
     ref = lat.to_host()
-    r_orc = orc.multilevel(og, ref, 3, 0, 4, 0, 1, 3, orc.metro(1, 4, 0.5), 404, call_id=3, mh_mode=1)
+    r_orc, field = orc.multilevel(og, ref, 3, 0, 4, 0, 1, 3, orc.metro(1, 4, 0.5), 404, call_id=3, mh_mode=1,
+                                  field=True)
     assert np.allclose(r64, r_orc, rtol=1e-11, atol=1e-20)
+    SV = dims[0] * dims[1] * dims[2]
+    scale = np.abs(field).reshape(3, dims[0] // 2, SV).mean(axis=(0, 2))[:10]
     f32 = lat.astype(u1.F32)
     r32 = f32.multilevel(3, 0, 4, 0, 1, 3, u1.BENCH_METRO, 404, call_id=3, mh_mode=1)
     r32_64 = orc.multilevel(og, f32.astype(u1.F64).to_host(), 3, 0, 4, 0, 1, 3, orc.metro(1, 4, 0.5), 404,
                             call_id=3, mh_mode=1)
-    scale = abs(r_orc[0])
     st = np.abs(r32 - r_orc)[:10] / scale
     kr = np.abs(r32 - r32_64)[:10] / scale
-    say(f"multilevel 24^4 fixed config |FP32 - FP64| / C(1), R=1..10: storage max {st.max():.2e} "
-        f"({np.array2string(st, precision=1)}), kernel max {kr.max():.2e}; C(1..3) = "
-        f"{np.array2string(r_orc[:3].real, precision=6)}")
+    say(f"multilevel 24^4 fixed config |FP32 - FP64| / S(R), R=1..10: storage max {st.max():.2e}, kernel max "
+        f"{kr.max():.2e}; |C(R)| / S(R) = {np.array2string(np.abs(r_orc[:10]) / scale, precision=1)}")
     assert st.max() <= 1e-6
     assert kr.max() <= 1e-6
 
@@ -215,38 +227,72 @@ def test_ensemble_64_uneven_fp64_parity(u1, orc):
 
 
 def test_ensemble_64_uneven_fp32_vs_fp64(u1):
-    """Timed: 300 graph sweeps of all 64 replicas with every replica's
-    plaquette summed inside each sweep (u1_ensemble_sweep_plaquette); the
-    series gives the per-replica <P> compared between precisions."""
+    """Timed: graph sweeps of all 64 replicas with every replica's plaquette
+    summed inside each sweep (u1_ensemble_sweep_plaquette); 500 thermalisation
+    + 2000 measured sweeps per replica.  Per replica, FP32 - FP64 <P> in units
+    of the combined error (the larger of the Gamma-method and the 10-bin
+    jackknife error, so that modes slower than the Gamma window still count):
+    median |z| < 1, at least 85 % within 2 sigma, none beyond 5 sigma."""
+    from latticeu1_amd.analysis import mean_error
+
     dims, betas, hot, seeds = ensemble_specs()
     geoms = [u1.geom(d, b) for d, b in zip(dims, betas)]
     n_links = sum(4 * int(np.prod(d)) for d in dims)
     series = {}
+    nmeas = 2000
     for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
         e = u1.Ensemble(geoms, seeds, hot, prec, 1, 3, u1.BENCH_METRO)
-        e.sweep(200, graph=True)  # thermalisation
+        e.sweep(500, graph=True)  # thermalisation
         e.sweep_plaquette(1)      # builds the measuring graph outside the timed region
         u1.sync()
         t0 = time.perf_counter()
-        ser = e.sweep_plaquette(300)
+        ser = e.sweep_plaquette(nmeas)
         el = time.perf_counter() - t0
-        say(f"ensemble {name}: 64 replicas, {n_links} links, 300 graph sweeps with per-sweep plaquettes in "
-            f"{el:.3f} s -> {4 * n_links * 300 / el:.3e} link updates/s")
+        say(f"ensemble {name}: 64 replicas, {n_links} links, {nmeas} graph sweeps with per-sweep plaquettes in "
+            f"{el:.3f} s -> {4 * n_links * nmeas / el:.3e} link updates/s")
         series[name] = np.array([[u1.mean_plaquette(row) for row in sw] for sw in ser])
         e.destroy()
-    from latticeu1_amd.analysis import mean_error
-
     z = []
     for r in range(len(dims)):
-        m32, e32 = mean_error(series["fp32"][:, r])
-        m64, e64 = mean_error(series["fp64"][:, r])
+        m32, e32 = mean_error(series["fp32"][:, r], 10)
+        m64, e64 = mean_error(series["fp64"][:, r], 10)
         z.append((m32 - m64) / math.hypot(e32, e64))
     z = np.array(z)
-    say(f"ensemble per-replica <P> FP32 - FP64 in combined sigma (Gamma-method errors): median |z| "
-        f"{np.median(np.abs(z)):.2f}, max |z| {np.max(np.abs(z)):.2f}, within 1 sigma {np.mean(np.abs(z) < 1):.2f}, "
-        f"within 2 sigma {np.mean(np.abs(z) < 2):.2f}")
-    # 300 sweeps per chain; replicas with beta near the transition (~1.01)
-    # tunnel slowly, so the bar is robust: a broken FP32 path shifts the bulk
-    assert np.median(np.abs(z)) < 1.2
-    assert np.mean(np.abs(z) < 2.0) >= 0.8
-    assert np.max(np.abs(z)) < 8.0
+    say(f"ensemble per-replica <P> FP32 - FP64 in combined sigma: median |z| {np.median(np.abs(z)):.2f}, max |z| "
+        f"{np.max(np.abs(z)):.2f}, within 1 sigma {np.mean(np.abs(z) < 1):.2f}, within 2 sigma "
+        f"{np.mean(np.abs(z) < 2):.2f}")
+    for r in np.argsort(-np.abs(z))[:3]:
+        say(f"  replica {r}: {dims[r]} beta {betas[r]:.4f} {'hot' if hot[r] else 'cold'} start, z {z[r]:+.2f}; "
+            f"200-sweep block means fp32 {np.array2string(series['fp32'][:, r].reshape(10, -1).mean(1), precision=4)}"
+            f" fp64 {np.array2string(series['fp64'][:, r].reshape(10, -1).mean(1), precision=4)}")
+    assert np.median(np.abs(z)) < 1.0
+    assert np.mean(np.abs(z) < 2.0) >= 0.85
+    assert np.max(np.abs(z)) < 5.0
+
+
+def test_small_volume_slow_modes_8x4_beta_1_081(u1):
+    """The largest per-replica deviation of the batched mix sits on an 8^4
+    replica at beta = 1.0811 (Coulomb phase), whose single chains drift
+    between long-lived states (block means move by several of their own
+    errors).  64 independent chains per precision at that point: the spread
+    of the chain means exceeds what the per-chain errors predict (slow
+    modes), while the FP32 and FP64 means over the chains agree within 3.5
+    combined sigma (errors from the chain-to-chain spread)."""
+    from latticeu1_amd.analysis import mean_error
+
+    geoms = [u1.geom((8, 8, 8, 8), 1.0811)] * 64
+    out = {}
+    for prec, name, off in ((u1.F32, "fp32", 0), (u1.F64, "fp64", 1000)):
+        e = u1.Ensemble(geoms, [7001 + off + i for i in range(64)], [1] * 64, prec, 1, 3, u1.BENCH_METRO)
+        e.sweep(300, graph=True)
+        ser = np.array([[u1.mean_plaquette(row) for row in sw] for sw in e.sweep_plaquette(1000)])
+        e.destroy()
+        chain = [mean_error(ser[:, i], 10) for i in range(64)]
+        means = np.array([c[0] for c in chain])
+        per_chain_err = float(np.sqrt(np.mean([c[1] ** 2 for c in chain])))
+        out[name] = (means.mean(), means.std(ddof=1) / 8.0, means.std(ddof=1) / per_chain_err)
+        say(f"8^4 beta 1.0811 {name}: <P> = {out[name][0]:.6f} +- {out[name][1]:.1e} over 64 chains; chain-mean "
+            f"spread / per-chain error = {out[name][2]:.2f}")
+    d = (out["fp32"][0] - out["fp64"][0]) / math.hypot(out["fp32"][1], out["fp64"][1])
+    say(f"8^4 beta 1.0811 FP32 - FP64 over chains: {d:+.2f} sigma")
+    assert abs(d) < 3.5

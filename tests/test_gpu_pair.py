@@ -101,18 +101,20 @@ def test_fp32_pair_matches_two_launches(u1, dims, tune):
     assert np.max(np.abs(out[0][2] - out[2][2])) <= 1e-12
 
 
-@pytest.mark.parametrize("block,spt", [(384, 3), (576, 2), (320, 4), (576, 1), (1024, 1), (192, 4), (128, 3)])
-def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt):
+@pytest.mark.parametrize("block,spt,bz", [(384, 3, 2), (576, 2, 2), (384, 3, 1), (576, 2, 3), (320, 4, 2),
+                                          (576, 1, 2), (1024, 1, 4), (192, 4, 2), (128, 3, 2)])
+def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt, bz):
     """Every fused-pair shape the tuner can choose on a 48 x 48 plane
-    (P = 1152 sites per parity: one exact step of 384 x 3 or 576 x 2, one
-    step with idle slots, two or more steps), forced through the tune cache,
+    (P = 1152 sites per parity: one exact step of 384 x 3 or 576 x 2; one
+    step with idle slots; two or more steps), in lexicographic, XCD-ordered
+    and tiled plane orders (block.z 1 to 4), forced through the tune cache,
     bit for bit against two launches in binary32."""
     dims = (48, 48, 4, 6)
     vol = "x".join(map(str, dims))
     cache = tmp_path / "shapes.csv"
-    rows = [f"{vol}\t{name}_f32\tthreads=halfvolume\t{b}\t{s}\t2\t0\t1\t1\t0\t# forced"
-            for name, b, s in (("Metropolis", 256, 1), ("OverRelaxation", 256, 1),
-                               ("MetropolisPair", block, spt), ("OverRelaxationPair", block, spt))]
+    rows = [f"{vol}\t{name}_f32\tthreads=halfvolume\t{b}\t{s}\t{z}\t0\t1\t1\t0\t# forced"
+            for name, b, s, z in (("Metropolis", 256, 1, 2), ("OverRelaxation", 256, 1, 2),
+                                  ("MetropolisPair", block, spt, bz), ("OverRelaxationPair", block, spt, bz))]
     cache.write_text("tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n" + "\n".join(rows) + "\n")
     g = u1.geom(dims, 1.0)
     out = {}
@@ -129,7 +131,7 @@ def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt):
     finally:
         u1.set_pair_fusion(1)
         u1.tune_enable(True)
-    say(f"fp32 pair shape {block} x {spt} vs two launches: accepts {out[2][1]}")
+    say(f"fp32 pair shape {block} x {spt} (block.z {bz}) vs two launches: accepts {out[2][1]}")
     assert np.array_equal(out[0][0], out[2][0])
     assert out[0][1] == out[2][1]
     assert np.max(np.abs(out[0][2] - out[2][2])) <= 1e-12
