@@ -69,6 +69,17 @@ int main(int argc, char** argv) {
   std::ofstream out("lattice.bin", std::ios::binary);
   out.write((const char*)h1.getPtr(), h1.Size() * sizeof(double));
   out.close();
+  // host-array passes (update.cu:51-98): Metropolis, overrelaxation, twice,
+  // on a host copy of the final lattice
+  PARAMS::accept_ratio = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    metropolis(h1.getPtr());
+    overrelaxation(h1.getPtr());
+  }
+  std::cout << "HOSTACCEPT " << PARAMS::accept_ratio << std::endl;
+  std::ofstream out2("host_passes.bin", std::ios::binary);
+  out2.write((const char*)h1.getPtr(), h1.Size() * sizeof(double));
+  out2.close();
   delete lat32;
   delete no;
   delete eo;

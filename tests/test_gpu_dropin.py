@@ -48,7 +48,27 @@ def test_dropin_driver_matches_oracle(orc, tmp_path):
     assert abs(complex(*vals["FIELD0"]) - means[0]) < 1e-12
     hp = orc.plaquette_host(og, ref)
     assert np.allclose(vals["HOSTPLAQ"], hp, atol=1e-12)
-    assert 0.2 < 1 - 0.5 * (vals["PLAQ32"][0] + vals["PLAQ32"][1]) < 0.9
+    # host-array metropolis() / overrelaxation() (update.cu:51-98) from the
+    # final lattice, host generator (seed 1234, sweeps 0, 1, 2, 3)
+    host = ref.copy()
+    acc = 0
+    for s in range(2):
+        acc += orc.update(og, host, 1, 0, m, 1234, 2 * s, count_accepts=True)
+        orc.update(og, host, 0, 1, m, 1234, 2 * s + 1)
+    got = np.fromfile(tmp_path / "host_passes.bin", dtype=np.float64)
+    assert np.array_equal(got.view(np.uint64), host.view(np.uint64))
+    assert vals["HOSTACCEPT"] == [float(acc)]
+    # FP32 overloads: the same kernels through the C ABI from Python (hot
+    # start, 5 sweeps of seed 1234) give the driver's FP32 plaquette
+    import latticeu1_amd as u1
+
+    u1.init(0)
+    l32 = u1.Lattice.cold(u1.geom(dims, 1.0, 1.0), u1.F32)
+    l32.hotstart(1234)
+    for s in range(5):
+        l32.update(1, 3, u1.BENCH_METRO, 1234, s)
+    p32 = l32.plaquette()
+    assert abs(vals["PLAQ32"][0] - p32[0]) <= 1e-14 and abs(vals["PLAQ32"][1] - p32[2]) <= 1e-14
     name = "8_8_8_8_beta_1_xi_1_mtr_1_ovr_3_iter_20"
     for fn in (f"Pot_{name}.dat", f"Pot_mhit_{name}.dat", f"Pot_mlevel_{name}_2_1_2_1_1_3.dat"):
         rows = (tmp_path / fn).read_text().strip().splitlines()
@@ -96,6 +116,16 @@ def test_run_driver_files_match_oracle(orc, tmp_path):
         f = row.split("\t")
         assert int(f[0]) == r + 1
         assert abs(complex(float(f[1]), float(f[2])) - c0[r]) < 1e-11 * max(1.0, abs(c0[r]))
-    for fn in (f"Pot_mhit_{name}_iter_1000.dat", f"Pot_mlevel_{name}_iter_1000_10_1_20_1_2_5.dat"):
+    # Pot_mhit_ (analytic multihit, polyakov.cu:634-650) and Pot_mlevel_
+    # (MultiLevel(10, 1, 20, 1, 2, 5), the run's first call: call id 0,
+    # multilevel.cu:712-730) against the oracle on the same configuration
+    want = {f"Pot_mhit_{name}_iter_1000.dat": orc.poly2(og, ref, 1),
+            f"Pot_mlevel_{name}_iter_1000_10_1_20_1_2_5.dat":
+                orc.multilevel(og, ref, 10, 1, 20, 1, 2, 5, m, 1234, call_id=0, mh_mode=1)}
+    for fn, c in want.items():
         rows = (tmp_path / fn).read_text().strip().splitlines()
-        assert len(rows) == 4 and rows[0].split("\t")[0] == "1", fn
+        assert len(rows) == 4, fn
+        for r, row in enumerate(rows):
+            f = row.split("\t")
+            assert int(f[0]) == r + 1
+            assert abs(complex(float(f[1]), float(f[2])) - c[r]) <= 1e-10 * max(abs(c[r]), 1e-12), (fn, r, row, c[r])
