@@ -128,6 +128,20 @@ int u1_phase(const u1_geom* g, int prec, void* lat, int32_t kind, int32_t parity
  * space-time plaquettes, normalised by 1/(3V) each (plaquette.cu:156-177).
  * Mean plaquette <P> = 1 - (out[0] + out[2]) / 2. */
 int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]);
+/* u1_plaquette that allocates nothing, for callers that cannot allocate
+ * device memory (code under hipGraph stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * result, in u1_plaquette's form, goes to out_dev (device memory, 4 doubles),
+ * asynchronously on the library stream; nothing synchronises.  The reduction
+ * rows (4 doubles per workgroup of the plaquette pass, plus 4 per 512 rows
+ * for the fold) live in a fixed 8 MiB device scratch array of the library
+ * (one per GPU, held until the process ends): every lattice up to 96^4 fits
+ * in both precisions; a lattice that needs more is rejected with U1_ERR_ARG
+ * before any launch.  The scratch keeps the last call's contents; every row
+ * is written before it is read, so nothing needs clearing.  Calls that
+ * overlap on one GPU share the scratch and race: issue them in order on one
+ * stream (the library stream does). */
+int u1_plaquette_noalloc(const u1_geom* g, int prec, const void* lat, double* out_dev);
 /* out[2] = Polyakov loop (1/SV) sum_x e^{i sum_t theta_t(x,t)}. */
 int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]);
 /* Per spatial site P(x) into P_dev (2*SV doubles, lexicographic spatial index).

@@ -84,6 +84,7 @@ _SIGS = {
     "u1_selftest_philox": (C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int64),
     "u1_selftest_stream": (C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)),
     "u1_plaquette": (C.POINTER(Geom), C.c_int, _P, C.POINTER(C.c_double)),
+    "u1_plaquette_noalloc": (C.POINTER(Geom), C.c_int, _P, _P),
     "u1_polyakov": (C.POINTER(Geom), C.c_int, _P, C.POINTER(C.c_double)),
     "u1_poly_volume": (C.POINTER(Geom), C.c_int, _P, C.c_int32, C.POINTER(Metro), C.c_uint64, C.c_uint32, C.c_uint32,
                        C.c_uint32, _P),

@@ -22,6 +22,17 @@ namespace {
 
 thread_local std::string g_err;
 
+// Scratch of u1_plaquette_noalloc: the reduction rows of the plaquette pass
+// and the fold's second-level rows, in a file-scope device array (each GPU the
+// library runs on holds its own copy until the process ends).  Host code never
+// passes the array's name to a kernel: it uses the device address that
+// hipGetSymbolAddress returns for the current device (kept per device in
+// g_scratch_dev, set in u1_init).
+constexpr size_t kNoAllocScratchDoubles = 1u << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+
 // Launch shape of an update phase kernel: workgroup size, even/odd sites per
 // thread, XCD-aware workgroup order.
 struct Cfg {
@@ -1001,6 +1012,11 @@ int u1_init(int device) {
   HIPCHK(hipStreamCreateWithFlags(&G.st, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void**)&G.counter, sizeof(unsigned long long)));
   HIPCHK(hipMalloc((void**)&G.dres, 64 * sizeof(double)));
+  if (device < kMaxDevices && !g_scratch_dev[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = (double*)p;
+  }
   G.device = device;
   G.init = true;
   return U1_OK;
@@ -1171,6 +1187,30 @@ int u1_plaquette(const u1_geom* g, int prec, const void* lat, double out[4]) {
   RET(check_prec(prec));
   RET(plaquette_async(g, prec, lat, G.dres));
   return copy_back(out, G.dres, 4);
+}
+
+int u1_plaquette_noalloc(const u1_geom* g, int prec, const void* lat, double* out_dev) {
+  REQ_INIT();
+  REQ_PTR(lat);
+  REQ_PTR(out_dev);
+  RET(check_geom(g));
+  RET(check_prec(prec));
+  double* rows = G.device < kMaxDevices ? g_scratch_dev[G.device] : nullptr;
+  if (!rows) return fail(U1_ERR_STATE, "no-alloc scratch is not available on this device");
+  RepDesc r = make_rep(g, 0, 0);
+  int block, spt;
+  plaquette_shape(r, prec, &block, &spt);
+  set_blocks(r, block * spt);
+  r.nblk *= 2;  // even sites, then odd sites
+  const int nblk = r.nblk;
+  const size_t need = 4 * (size_t)nblk + u1d::fold_scratch(nblk, 4);
+  if (need > kNoAllocScratchDoubles)
+    return fail(U1_ERR_ARG, "lattice too large for u1_plaquette_noalloc: needs " + std::to_string(need) +
+                                " scratch doubles, the maximum is " + std::to_string(kNoAllocScratchDoubles));
+  // every row and every fold row is written before it is read: nothing to clear
+  HIPCHK(u1d::launch_plaquette(prec, lat, r, rows, nblk, block, spt, G.st));
+  HIPCHK(u1d::launch_finish_sums(rows, nblk, 4, 1.0 / (3.0 * (double)r.V), out_dev, rows + 4 * (size_t)nblk, G.st));
+  return U1_OK;
 }
 
 int u1_polyakov(const u1_geom* g, int prec, const void* lat, double out[2]) {
