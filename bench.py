@@ -183,6 +183,7 @@ def run_batched(u1, steps, warmup, skip_fp64):
         if warmup > 0:
             e.sweep(warmup, graph=True)  # thermalisation (also runs the ensemble tuner)
         e.sweep_plaquette(1)  # builds the measuring graph outside the timed region
+        u1.stream_rate(0, 1024, reps=1)  # kernel-trace marker: the timed sweeps follow
         ms, kms, series = e.bench(steps, kernel_timing=True)
         s = 4 if prec == u1.F32 else 8
         cls = class_table(kms, vol, s)
@@ -220,6 +221,36 @@ def cpu_baseline(dims, beta, seed, budget_s):
             "sample": f"{n} sweep(s) of the same 48^4 workload (FP64 OpenMP oracle, hot start), {el:.1f} s"}
 
 
+def run_multilevel(u1, levels, warmup, eager, skip_fp64):
+    """The multilevel workload: 24^4, beta = 1, hot start, `warmup` sweeps,
+    then MultiLevel(n4 = n2 = levels, k4 = k2 = 1, 1 Metropolis + 3 OR per
+    sublattice sweep) per precision, timed on the host around the call (which
+    returns after its device work).  A short call first settles the launch
+    shapes, and one streaming kernel then marks the start of the timed call
+    in a kernel trace."""
+    u1.set_graph_capture(not eager)
+    out = {"workload": "multilevel 24^4, beta=1.0, n4=n2=%d, k4=k2=1, 1 Metropolis + 3 OR per sublattice sweep, "
+                       "analytic multihit, R=1..12" % levels,
+           "level2_steps": "graph replays" if not eager else "eager launches"}
+    for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):
+        if prec == u1.F64 and skip_fp64:
+            continue
+        g = u1.geom((24, 24, 24, 24), 1.0)
+        lat = u1.Lattice.cold(g, prec)
+        lat.hotstart(404)
+        for s in range(warmup):
+            lat.update(1, 3, u1.BENCH_METRO, 404, s)
+        lat.multilevel(1, 1, 1, 1, 1, 3, u1.BENCH_METRO, 404, call_id=9)  # settles the launch shapes
+        u1.stream_rate(0, 1024, reps=1)  # trace marker
+        u1.sync()
+        t0 = time.perf_counter()
+        c = lat.multilevel(levels, 1, levels, 1, 1, 3, u1.BENCH_METRO, 404, call_id=0)
+        el = time.perf_counter() - t0
+        out[name] = {"seconds_per_call": el, "C_R": [[float(v.real), float(v.imag)] for v in c]}
+        print(f"[bench] multilevel {name}: {el:.3f} s per call; C(1) = {c[0]:.6e}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +264,12 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="print every autotuner candidate (stderr)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-batched", action="store_true", help="leave out the 64-replica batched ensemble")
+    ap.add_argument("--mode", choices=["headline", "batched", "multilevel"], default="headline",
+                    help="headline: the 48^4 line (default); batched: only the 64-replica mix; multilevel: only the "
+                         "24^4 multilevel call")
+    ap.add_argument("--ml-levels", type=int, default=50, help="multilevel mode: n4 = n2")
+    ap.add_argument("--ml-eager", action="store_true",
+                    help="multilevel mode: enqueue the level-2 steps eagerly instead of replaying one graph")
     ap.add_argument("--plaq", choices=["fused", "separate"], default="fused",
                     help="plaquette every sweep: summed inside the last overrelaxation pass (fused) or by a "
                          "separate pass over the lattice")
@@ -266,6 +303,15 @@ def main():
     u1.set_pair_fusion(args.pair_fusion)
     if args.tunecache and os.path.exists(args.tunecache):
         u1.tune_load(args.tunecache)
+    if args.mode == "batched":
+        print(json.dumps({"mode": "batched", "device": dev,
+                          "batched": run_batched(u1, args.steps, args.warmup, args.skip_fp64)}))
+        return
+    if args.mode == "multilevel":
+        print(json.dumps({"mode": "multilevel", "device": dev,
+                          "multilevel": run_multilevel(u1, args.ml_levels, args.warmup, args.ml_eager,
+                                                       args.skip_fp64)}))
+        return
 
     res = {}
     for prec, name in ((u1.F32, "fp32"), (u1.F64, "fp64")):

@@ -239,7 +239,7 @@ int u1_set_graph_capture(int32_t on);
 int u1_set_verbosity(int32_t level);
 int u1_tune_load(const char* path);
 int u1_tune_save(const char* path);
-/* Number of tuned entries; if key/val are given, entry i as text. */
+/* Number of launch-shape entries held (tuned in this process or loaded). */
 int u1_tune_entries(int32_t* n);
 
 /* ---------------------------------------------------------------- timing helper */

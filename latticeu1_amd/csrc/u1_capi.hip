@@ -1308,8 +1308,9 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   if (rmax > 32) return fail(U1_ERR_ARG, "Nx/2 must be <= 32");
   const int64_t sz = (int64_t)r.SV * rmax * 3;
   const int64_t nl2 = r.N[3] / 2, nl4 = r.N[3] / 4;
+  if (sz * nl2 >= ((int64_t)1 << 31)) return fail(U1_ERR_ARG, "lattice too large for the multilevel T2 tensor");
   const size_t latb = 4 * (size_t)r.V * elem_size(prec);
-  void *wl = nullptr, *tl = nullptr, *l2 = nullptr, *l4 = nullptr;
+  void *wl = nullptr, *tl = nullptr, *pq = nullptr, *l2 = nullptr, *l4 = nullptr;
   uint32_t* ctr = nullptr;  // level-2 step counter i * n2 + k, advanced by the graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1320,6 +1321,7 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
     if (graph) (void)hipGraphDestroy(graph);
     if (wl) (void)hipFree(wl);
     if (tl) (void)hipFree(tl);
+    if (pq) (void)hipFree(pq);
     if (l2) (void)hipFree(l2);
     if (l4) (void)hipFree(l4);
     if (ctr) (void)hipFree(ctr);
@@ -1334,6 +1336,7 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
   } while (0)
   MLCHK(hipMalloc(&wl, latb));
   MLCHK(hipMalloc(&tl, 2 * (size_t)r.V * sizeof(double)));
+  MLCHK(hipMalloc(&pq, 4 * (size_t)r.SV * (size_t)nl2 * sizeof(double)));
   MLCHK(hipMalloc(&l2, 2 * (size_t)sz * nl2 * sizeof(double)));
   MLCHK(hipMalloc(&l4, 2 * (size_t)sz * nl4 * sizeof(double)));
   MLCHK(hipMemcpyAsync(wl, lat, latb, hipMemcpyDeviceToDevice, G.st));
@@ -1366,7 +1369,7 @@ int u1_multilevel(const u1_geom* g, int prec, const void* lat, int32_t n4, int32
                      nullptr, ctr, (uint32_t)k2));
     hipError_t e = u1d::launch_temporal_links(prec, wl, r, mh_mode, mp, mn, ms, u1r::STREAM_ML_MULTIHIT, 0, call_id,
                                               (double*)tl, 256, G.st, ctr);
-    if (e == hipSuccess) e = u1d::launch_ml_l2((const double*)tl, (double*)l2, r, rmax, 256, G.st);
+    if (e == hipSuccess) e = u1d::launch_ml_l2((const double*)tl, (double*)pq, (double*)l2, r, rmax, 256, G.st);
     if (e == hipSuccess) e = u1d::launch_increment(ctr, G.st);
     if (e != hipSuccess) return fail(U1_ERR_HIP, std::string("multilevel level-2 step: ") + hipGetErrorString(e));
     return U1_OK;

@@ -92,7 +92,10 @@ hipError_t launch_plaquette_fields(int prec, const void* lat, const RepDesc& r, 
 hipError_t launch_fill_zero(void* p, int64_t bytes, hipStream_t st);
 
 // multilevel (u1_multilevel.hip)
-hipError_t launch_ml_l2(const double* tl, double* l2, const RepDesc& r, int rmax, int block, hipStream_t st);
+// T2 accumulation of one level-2 step; pq: scratch of 4 SV (Nt/2) doubles for
+// the two-slice products.
+hipError_t launch_ml_l2(const double* tl, double* pq, double* l2, const RepDesc& r, int rmax, int block,
+                        hipStream_t st);
 hipError_t launch_ml_l4(const double* l2, double* l4, const RepDesc& r, int rmax, double l2norm, int block,
                         hipStream_t st);
 hipError_t launch_ml_final(const double* l4, double* ppspace, const RepDesc& r, int rmax, double l4norm,

@@ -18,34 +18,51 @@ struct C64 {
 __device__ __forceinline__ C64 cmul(C64 a, C64 b) { return {a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im}; }
 __device__ __forceinline__ C64 ldc(const double* p, int64_t i) { return {p[2 * i], p[2 * i + 1]}; }
 
-// multilevel.cu:331-360: l2[x,R,dir,layer] += prod_{t in layer} P(x,t) *
-// prod_{t in layer} conj P(x + R e_dir, t); one thread per (x, R, dir).
-__global__ __launch_bounds__(256) void k_ml_l2(const double* __restrict__ tl, double* __restrict__ l2, RepDesc r,
-                                               int rmax) {
-  const int64_t sz = (int64_t)r.SV * rmax * 3;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The two-slice products of the temporal-link field, once per level-2 step:
+// p2[l SV + x] = prod_{t = 2l, 2l+1} P(x, t) and q2[l SV + x] = the same
+// product of the conjugates, each from 1 in the reference's operation order
+// (multilevel.cu:342-347), so k_ml_l2 needs two loads per layer instead of
+// four and no longer recomputes them for each of its 3 Rmax (R, dir).
+__global__ __launch_bounds__(256) void k_ml_pair2(const double* __restrict__ tl, double* __restrict__ p2,
+                                                  double* __restrict__ q2, RepDesc r) {
+  // 32-bit indices: the host checks SV * Rmax * 3 * Nt/2 < 2^31
+  const uint32_t n = (uint32_t)r.SV * (uint32_t)(r.N[3] / 2);
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const uint32_t l = idx / (uint32_t)r.SV, x = idx - l * (uint32_t)r.SV;
+  C64 p0 = {1.0, 0.0}, p1 = {1.0, 0.0};
+  for (uint32_t tt = 0; tt < 2; ++tt) {
+    const C64 q = ldc(tl, x + (2 * l + tt) * (uint32_t)r.SV);
+    p0 = cmul(p0, q);
+    p1 = cmul(p1, C64{q.re, -q.im});
+  }
+  p2[2 * idx] = p0.re;
+  p2[2 * idx + 1] = p0.im;
+  q2[2 * idx] = p1.re;
+  q2[2 * idx + 1] = p1.im;
+}
+
+// multilevel.cu:331-360: l2[x,R,dir,layer] += p2(x, layer) * q2(x + R e_dir,
+// layer); one thread per (x, R, dir).
+__global__ __launch_bounds__(256) void k_ml_l2(const double* __restrict__ p2, const double* __restrict__ q2,
+                                               double* __restrict__ l2, RepDesc r, int rmax) {
+  // 32-bit indices: the host checks SV * Rmax * 3 * Nt/2 < 2^31
+  const uint32_t SV = (uint32_t)r.SV, sz = SV * (uint32_t)rmax * 3u;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= sz) return;
-  const int32_t id = (int32_t)(idx % r.SV);
-  const int32_t R = (int32_t)((idx / r.SV) % rmax) + 1;
-  const int32_t dir = (int32_t)(idx / ((int64_t)r.SV * rmax));
-  int32_t x[3] = {id % r.N[0], (id / r.N[0]) % r.N[1], id / (r.N[0] * r.N[1])};
-  int32_t y[3] = {x[0], x[1], x[2]};
-  y[dir] = (y[dir] + R) % r.N[dir];
-  const int64_t s0 = x[0] + (int64_t)r.N[0] * (x[1] + (int64_t)r.N[1] * x[2]);
-  const int64_t s1 = y[0] + (int64_t)r.N[0] * (y[1] + (int64_t)r.N[1] * y[2]);
-  int layer = 0;
-  for (int t = 0; t < r.N[3]; t += 2) {
-    C64 p0 = {1.0, 0.0}, p1 = {1.0, 0.0};
-    for (int tt = t; tt < t + 2; ++tt) {
-      p0 = cmul(p0, ldc(tl, s0 + (int64_t)tt * r.SV));
-      const C64 q = ldc(tl, s1 + (int64_t)tt * r.SV);
-      p1 = cmul(p1, C64{q.re, -q.im});
-    }
-    const C64 pl = cmul(p0, p1);
-    const int64_t pos = idx + sz * layer;
+  const uint32_t q = idx / SV, id = idx - q * SV;
+  const uint32_t dir = q / (uint32_t)rmax, R = q - dir * (uint32_t)rmax + 1u;
+  const uint32_t n0 = (uint32_t)r.N[0], n1 = (uint32_t)r.N[1];
+  const uint32_t x1x2 = id / n0, x0 = id - x1x2 * n0, x2 = x1x2 / n1, x1 = x1x2 - x2 * n1;
+  uint32_t y[3] = {x0, x1, x2};
+  y[dir] = (y[dir] + R) % (uint32_t)r.N[dir];
+  const uint32_t s0 = id, s1 = y[0] + n0 * (y[1] + n1 * y[2]);
+  const uint32_t nl2 = (uint32_t)r.N[3] / 2u;
+  for (uint32_t layer = 0; layer < nl2; ++layer) {
+    const C64 pl = cmul(ldc(p2, s0 + layer * SV), ldc(q2, s1 + layer * SV));
+    const uint32_t pos = idx + sz * layer;
     l2[2 * pos] = pl.re + l2[2 * pos];
     l2[2 * pos + 1] = pl.im + l2[2 * pos + 1];
-    ++layer;
   }
 }
 
@@ -107,9 +124,14 @@ __global__ __launch_bounds__(256) void k_ml_final(const double* __restrict__ l4,
 
 }  // namespace
 
-hipError_t launch_ml_l2(const double* tl, double* l2, const RepDesc& r, int rmax, int block, hipStream_t st) {
+hipError_t launch_ml_l2(const double* tl, double* pq, double* l2, const RepDesc& r, int rmax, int block,
+                        hipStream_t st) {
+  const int64_t n = (int64_t)r.SV * (r.N[3] / 2);
+  double* p2 = pq;
+  double* q2 = pq + 2 * n;
+  k_ml_pair2<<<(int)((n + block - 1) / block), block, 0, st>>>(tl, p2, q2, r);
   const int64_t sz = (int64_t)r.SV * rmax * 3;
-  k_ml_l2<<<(int)((sz + block - 1) / block), block, 0, st>>>(tl, l2, r, rmax);
+  k_ml_l2<<<(int)((sz + block - 1) / block), block, 0, st>>>(p2, q2, l2, r, rmax);
   return hipGetLastError();
 }
 
