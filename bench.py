@@ -56,23 +56,39 @@ HBM_PEAK_GBS = 8000.0
 
 # launch classes of u1_bench_updates' kernel timing
 CLASSES = ("metropolis", "overrelaxation", "overrelaxation_with_plaquette", "metropolis_pair",
-           "overrelaxation_pair", "overrelaxation_pair_with_plaquette")
-PAIR = (False, False, False, True, True, True)
+           "overrelaxation_pair", "overrelaxation_pair_with_plaquette", "metropolis_pass", "overrelaxation_pass",
+           "overrelaxation_pass_with_plaquette")
+NCLS = len(CLASSES)
+# phases per launch: 1 (phase), 2 (fused pair), 4 (fused parity pass)
+PHASES = (1, 1, 1, 2, 2, 2, 4, 4, 4)
 KERNEL_NAMES = ("k_phase<float> Metropolis (parity, mu) phase",
                 "k_phase<float> overrelaxation (parity, mu) phase",
                 "k_phase<float, MEAS> overrelaxation phase summing the plaquette",
                 "k_pair<float> fused Metropolis (parity, 0) + (parity, 1)",
                 "k_pair<float> fused overrelaxation (parity, 0) + (parity, 1)",
-                "k_pair<float, MEAS> fused overrelaxation pair summing the plaquette")
+                "k_pair<float, MEAS> fused overrelaxation pair summing the plaquette",
+                "k_chain<float> fused Metropolis parity pass (phases 0-3)",
+                "k_chain<float> fused overrelaxation parity pass (phases 0-3)",
+                "k_chain<float, MEAS> fused overrelaxation parity pass summing the plaquette")
 # kernel-trace names of each class (template arguments: phase <Real, MU, KIND,
-# BATCHED, SPT, MEAS>, pair <Real, KIND, SPT, MEAS, BATCHED>; KIND 0 Metropolis,
-# 1 overrelaxation)
+# BATCHED, SPT, MEAS>, pair <Real, KIND, SPT, MEAS, BATCHED>, pass <Real, KIND,
+# SPT, MEAS>; KIND 0 Metropolis, 1 overrelaxation)
 TRACE_PATTERNS = (r"k_phase<float, \d, 0, (false|true), \d, false>",
                   r"k_phase<float, \d, 1, (false|true), \d, false>",
                   r"k_phase<float, \d, 1, (false|true), \d, true>",
                   r"k_pair<float, 0, \d, false(, (false|true))?>",
                   r"k_pair<float, 1, \d, false(, (false|true))?>",
-                  r"k_pair<float, 1, \d, true(, (false|true))?>")
+                  r"k_pair<float, 1, \d, true(, (false|true))?>",
+                  r"k_chain<float, 0, \d, false>",
+                  r"k_chain<float, 1, \d, false>",
+                  r"k_chain<float, 1, \d, true>")
+
+
+def algorithmic_bytes(c, V, s):
+    """Compulsory bytes of one launch of class c: every phase reads the whole
+    lattice (4V angles) and writes the V/2 links it updates, so a launch of k
+    phases moves (4 + k/2) V angles: 4.5 V (phase), 5 V (pair), 6 V (pass)."""
+    return (4.0 + 0.5 * PHASES[c]) * V * s
 
 
 def ensemble_specs(seed=2024, n=64):
@@ -105,12 +121,12 @@ def class_table(kms, V, s):
     time per sweep and its share of the sum over classes."""
     out = {}
     for c, name in enumerate(CLASSES):
-        if kms[6 + c] > 0:
-            b = (5.0 if PAIR[c] else 4.5) * V * s
+        if kms[NCLS + c] > 0:
+            b = algorithmic_bytes(c, V, s)
             t = float(kms[c])
-            out[name] = {"mean_launch_ms": round(t, 5), "per_sweep": round(float(kms[6 + c]), 3),
+            out[name] = {"mean_launch_ms": round(t, 5), "per_sweep": round(float(kms[NCLS + c]), 3),
                          "bytes_per_launch": b, "achieved_GBs": round(b / (t * 1e-3) / 1e9, 1),
-                         "ms_per_sweep": round(t * float(kms[6 + c]), 5)}
+                         "ms_per_sweep": round(t * float(kms[NCLS + c]), 5)}
     tot = sum(v["ms_per_sweep"] for v in out.values())
     for v in out.values():
         v["time_share"] = round(v["ms_per_sweep"] / tot, 4) if tot > 0 else None
@@ -282,6 +298,9 @@ def main():
     ap.add_argument("--pair-fusion", type=int, choices=[0, 1, 2], default=1,
                     help="fused (parity, 0) + (parity, 1) update launches: 0 never, 1 where the tuner measures "
                          "them faster, 2 always")
+    ap.add_argument("--pass-fusion", type=int, choices=[0, 1, 2], default=1,
+                    help="fused parity passes (phases 0-3 in one launch): 0 never, 1 where the tuner measures "
+                         "them faster, 2 always")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the run, copy a buffer of this many MiB once (plain copy kernel of known size, "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -301,6 +320,7 @@ def main():
     if args.verbose:
         u1.set_verbosity(2)
     u1.set_pair_fusion(args.pair_fusion)
+    u1.set_pass_fusion(args.pass_fusion)
     if args.tunecache and os.path.exists(args.tunecache):
         u1.tune_load(args.tunecache)
     if args.mode == "batched":
@@ -337,11 +357,12 @@ def main():
     f32 = res["fp32"]
     cls = f32["cls"]
     for k, v in cls.items():
-        if PAIR[CLASSES.index(k)]:  # two phases' worth of per-phase algorithmic bytes
-            v["phase_equivalent_GBs"] = round(9.0 * V * f32["elem"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
+        nph = PHASES[CLASSES.index(k)]
+        if nph > 1:  # nph phases' worth of per-phase algorithmic bytes
+            v["phase_equivalent_GBs"] = round(4.5 * nph * V * f32["elem"] / (v["mean_launch_ms"] * 1e-3) / 1e9, 1)
     dom, roofline = roofline_of(cls, f32["ms"] / args.steps)
     achieved, d = roofline["achieved"], cls[dom]
-    roofline["kernel_timing"] = {"back_to_back_launches_per_class": int(f32["kms"][13]),
+    roofline["kernel_timing"] = {"back_to_back_launches_per_class": int(f32["kms"][2 * NCLS + 1]),
                                  "ms_per_sweep_sum_of_classes": round(sum(v["ms_per_sweep"] for v in cls.values()), 4),
                                  "ms_per_sweep_timed_region": round(f32["ms"] / args.steps, 4)}
 

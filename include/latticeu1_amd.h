@@ -201,10 +201,10 @@ int u1_ensemble_sweep_plaquette(u1_ensemble* e, int32_t nsweeps, double* series)
 /* Timing helper (no counterpart in the reference): u1_ensemble_sweep_plaquette
  * over nsweeps (>= 1; ovrn >= 1) with elapsed_ms the device time from one HIP
  * event pair on the library stream around the graph replays (series copies
- * of all but the last chunk included).  kernel_ms (nullable, 14 doubles) as in
+ * of all but the last chunk included).  kernel_ms (nullable, 20 doubles) as in
  * u1_bench_updates, for the ensemble's launches over all replicas: classes
  * 0 Metropolis, 1 overrelaxation, 2 measuring overrelaxation phase, 3 fused
- * Metropolis pair, 4 fused overrelaxation pair (5 unused), timed on a scratch
+ * Metropolis pair, 4 fused overrelaxation pair (5-8 unused), timed on a scratch
  * copy of the packed lattice after the timed region. */
 int u1_ensemble_bench(u1_ensemble* e, int32_t nsweeps, double* series, double* elapsed_ms, double* kernel_ms);
 /* out[4*nrep]: per-replica plaquette as u1_plaquette. */
@@ -227,6 +227,19 @@ int u1_tune_enable(int32_t on);
  * always.  The tune cache keeps the decision as "MetropolisPair" /
  * "OverRelaxationPair" entries (block.x 0 = two launches). */
 int u1_set_pair_fusion(int32_t mode);
+/* Fused parity passes (no reference counterpart): on a single lattice without
+ * multilevel mask whose (z, t) planes tile 2 x 4 (Nz even, Nt a multiple of
+ * 4), the four phases (parity, 0..3) of a pass can run as one launch; the
+ * workgroups of phase mu wait on device-side completion counters of the
+ * phase-(mu - 1) planes they depend on instead of a launch boundary.  The
+ * results are bit-identical to the separate launches.  mode 0: never; 1
+ * (default): where the tuner measures the fused pass faster (needs tuning
+ * on); 2: always.  Not used during stream capture.  The tune cache keeps the
+ * decision as "MetropolisChain" / "OverRelaxationChain" entries (block.x 0 =
+ * unfused).  A dependency wait that times out (bounded, about 0.1 s) is
+ * reported as U1_ERR_HIP by the next call that synchronises the library
+ * stream (u1_sync, or any call returning results to the host). */
+int u1_set_pass_fusion(int32_t mode);
 /* Multilevel level-2 steps (u1_multilevel): replayed from one captured
  * hipGraph (on = 1, default) or enqueued eagerly (0).  The same kernels read
  * the same device-side step counter either way, so the results are
@@ -248,15 +261,17 @@ int u1_tune_entries(int32_t* n);
  * into the sweep as u1_update_plaquette, 2 a separate u1_plaquette pass) and
  * return the elapsed device time in ms, from HIP events on the library stream.
  * The timed region holds no other events.
- * kernel_ms (nullable, 14 doubles): kernel timing after the timed region, on
+ * kernel_ms (nullable, 20 doubles): kernel timing after the timed region, on
  * a scratch copy of the lattice (the chain is not advanced): per launch class
  * c of the sweep -- 0 Metropolis phase, 1 overrelaxation phase, 2 measuring
  * overrelaxation phase, 3 fused Metropolis pair, 4 fused overrelaxation pair,
- * 5 measuring fused overrelaxation pair (u1_set_pair_fusion) -- 32
- * back-to-back launches in the sweep's shapes between one event pair;
+ * 5 measuring fused overrelaxation pair (u1_set_pair_fusion), 6 fused
+ * Metropolis pass, 7 fused overrelaxation pass, 8 measuring fused
+ * overrelaxation pass (u1_set_pass_fusion) -- 32 back-to-back launches in the
+ * sweep's shapes between one event pair;
  * kernel_ms[c] is the mean launch duration (0 if the sweep has none) and
- * kernel_ms[6 + c] the launches per sweep; kernel_ms[12] is the timed
- * region's ms per sweep and kernel_ms[13] the launches timed per class. */
+ * kernel_ms[9 + c] the launches per sweep; kernel_ms[18] is the timed
+ * region's ms per sweep and kernel_ms[19] the launches timed per class. */
 int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovrn, const u1_metro* m,
                      uint64_t seed, uint32_t sweep0, int32_t nsweeps, int32_t measure_plaq, double* plaq_series,
                      double* elapsed_ms, double* kernel_ms);

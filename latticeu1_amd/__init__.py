@@ -112,6 +112,7 @@ _SIGS = {
     "u1_ensemble_destroy": (_P,),
     "u1_tune_enable": (C.c_int32,),
     "u1_set_pair_fusion": (C.c_int32,),
+    "u1_set_pass_fusion": (C.c_int32,),
     "u1_set_graph_capture": (C.c_int32,),
     "u1_set_verbosity": (C.c_int32,),
     "u1_tune_load": (C.c_char_p,),
@@ -359,12 +360,12 @@ class Lattice:
               kernel_timing: bool = False):
         """measure_plaq: 0 none, 1 fused into the sweep, 2 separate plaquette pass.
         kernel_timing: afterwards, time every launch class of the sweep on a
-        scratch copy (the chain is not advanced); returns the 14 values of
+        scratch copy (the chain is not advanced); returns the 20 values of
         u1_bench_updates' kernel_ms (else None)."""
         measure_plaq = int(measure_plaq)
         series = np.zeros(4 * nsweeps) if measure_plaq else None
         el = C.c_double(0)
-        km = np.zeros(14) if kernel_timing else None
+        km = np.zeros(20) if kernel_timing else None
         call("u1_bench_updates", C.byref(self.g), self.prec, self.buf.ptr, metrop, ovrn, C.byref(m), seed, sweep0,
              nsweeps, measure_plaq, _dp(series) if measure_plaq else None, C.byref(el),
              _dp(km) if kernel_timing else None)
@@ -403,7 +404,7 @@ class Ensemble:
         (ms, kernel_ms or None, series); kernel_ms as Lattice.bench."""
         out = np.zeros(4 * self.n * nsweeps)
         ms = C.c_double(0.0)
-        kms = np.zeros(14) if kernel_timing else None
+        kms = np.zeros(20) if kernel_timing else None
         call("u1_ensemble_bench", self.ptr, nsweeps, _dp(out), C.byref(ms), _dp(kms) if kernel_timing else None)
         return ms.value, kms, out.reshape(nsweeps, self.n, 4)
 
@@ -480,6 +481,13 @@ def set_pair_fusion(mode: int) -> None:
     """Fused (parity, 0) + (parity, 1) update launches on single lattices:
     0 never, 1 where the tuner measures them faster (default), 2 always."""
     call("u1_set_pair_fusion", mode)
+
+
+def set_pass_fusion(mode: int) -> None:
+    """Fused parity passes (the four phases of a parity in one launch, ordered
+    by device-side completion counters) on single lattices: 0 never, 1 where
+    the tuner measures them faster (default), 2 always."""
+    call("u1_set_pass_fusion", mode)
 
 
 def tune_save(path: str) -> None:

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/latticeu1_amd.h"
+#include "u1_chain.h"
 #include "u1_kernels.h"
 
 using u1d::PhaseArgs;
@@ -31,12 +32,15 @@ thread_local std::string g_err;
 constexpr size_t kNoAllocScratchDoubles = 1u << 20;  // 8 MiB
 static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 constexpr int kMaxDevices = 64;
+constexpr int kChainMaxMargin = 1024;  // fused-pass schedule margin accepted from a tune cache
+constexpr float kChainMinGain = 0.97f;  // a fused pass must take under 97 % of the unfused pass's time
 double* g_scratch_dev[kMaxDevices] = {};
 
 // Launch shape of an update phase kernel: workgroup size, even/odd sites per
 // thread, XCD-aware workgroup order.
 struct Cfg {
   int block, spt, xcd, tile = 0;  // tile: single-lattice planes in 2 x 4 tiles (set_phase_tiles)
+  int margin = 0;                 // fused passes (tile: tiles along z): schedule margin (chain_schedule)
 };
 
 struct Ctx {
@@ -57,6 +61,16 @@ struct Ctx {
   // multilevel level-2 steps: replayed from one captured hipGraph (true) or
   // enqueued eagerly (false); the same kernels and counters either way
   bool graphs = true;
+  // fused parity passes (k_chain) on single lattices: 0 never, 1 where the
+  // tuner measures them faster, 2 always (where the geometry allows)
+  int chain = 1;
+  uint32_t* chain_cnt = nullptr;  // completion counters, 4 Nz Nt
+  size_t chain_cnt_cap = 0;       // in counters
+  uint32_t* chain_err = nullptr;  // wait-timeout flag (pinned host memory, written by the kernels)
+  int chain_nz = 0, chain_nt = 0, chain_per = 0;  // layout of the counters' last use (per: wavefronts per plane)
+  uint32_t chain_inc[4] = {};                     // launches that counted into each row since the last clear
+  std::map<std::string, uint32_t*> chain_sched;   // "Nz Nt bpp" -> schedule list (device)
+  std::map<std::string, int> chain_sched_len;
   std::map<std::string, Cfg> tunecache;  // key -> launch shape
   int tune_loaded = 0;
 };
@@ -476,18 +490,282 @@ int pair_cfg(int prec, int kind, const u1_geom* g, const void* lat, const PhaseA
   return U1_OK;
 }
 
-// Settles (tunes, if enabled) every launch shape a sweep of this geometry
-// uses, so that a later sweep allocates and synchronises nothing (graph
-// capture, timed regions).
-int settle_update_cfgs(int prec, const u1_geom* g, const void* lat, const PhaseArgs& proto, int metrop, int ovrn) {
-  Cfg c, pc;
-  if (metrop > 0) {
-    RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, proto, &c));
-    RET(pair_cfg(prec, u1d::KIND_METROPOLIS, g, lat, proto, c, &pc));
+// ------------------------------------------------------------ fused passes
+// Reports, and clears, a wait timeout of a fused pass (k_chain) since the
+// last check; called after the library stream has been synchronised.
+int chain_check() {
+  if (G.chain_err && __atomic_load_n(G.chain_err, __ATOMIC_ACQUIRE) != 0) {
+    __atomic_store_n(G.chain_err, 0u, __ATOMIC_RELEASE);
+    return fail(U1_ERR_HIP, "fused parity pass: a dependency wait timed out; the updated lattice is not reliable");
   }
-  if (ovrn > 0) {
-    RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, proto, &c));
-    RET(pair_cfg(prec, u1d::KIND_OVERRELAX, g, lat, proto, c, &pc));
+  return U1_OK;
+}
+
+// Shapes of a fused pass: Cfg{block, spt, 1, tz, margin}, planes cut into
+// chunks of block * spt sites, the (z, t) planes in tz x (8 / tz) tiles (one
+// per XCD), dependencies at least margin + 1 schedule steps back.
+void set_chain_blocks(RepDesc& r, const Cfg& c) { u1d::set_phase_blocks(r, c.block * c.spt); }
+bool chain_shape_ok(const RepDesc& r0, const Cfg& c) {
+  if (!(c.spt == 1 || c.spt == 2) || c.block < 64 || c.block > 512 || c.block % 64 != 0 || c.margin < 0 ||
+      c.margin > kChainMaxMargin)
+    return false;
+  RepDesc r = r0;
+  set_chain_blocks(r, c);
+  return u1d::chain_possible(r.N[2], r.N[3], r.bpp, c.tile);
+}
+// Workgroup shapes (first tuning stage, default tiling and margin), then the
+// tilings and margins of the best shape (second stage).
+std::vector<Cfg> chain_shapes(const RepDesc& r) {
+  std::vector<Cfg> v;
+  for (int spt : {1, 2})
+    for (int b : {64, 128, 192, 256, 384, 512}) {
+      Cfg c{b, spt, 1, 2, 64};
+      if (!chain_shape_ok(r, c)) c.tile = 1;
+      if (chain_shape_ok(r, c)) v.push_back(c);
+    }
+  return v;
+}
+std::vector<Cfg> chain_layouts(const RepDesc& r, const Cfg& shape) {
+  std::vector<Cfg> v;
+  for (int tz : {1, 2, 4, 8})
+    for (int margin : {16, 32, 64, 128, 256}) {
+      Cfg c = shape;
+      c.tile = tz;
+      c.margin = margin;
+      if (chain_shape_ok(r, c)) v.push_back(c);
+    }
+  return v;
+}
+
+// Schedule list, completion counters and counter targets of a fused pass over
+// r (which carries set_chain_blocks) in workgroups of `block` threads;
+// *nblocks: its workgroups.  Every wavefront adds 1 to its (mu, plane)
+// counter, so a launch adds per = bpp * block / 64 to each.  The counters only
+// grow: a launch waits for the value the previous launches with the same
+// layout have brought them to plus per, and they are cleared (by a fill
+// kernel on the library stream) when the layout changes or before they could
+// overflow.
+int chain_prepare(const RepDesc& r, const Cfg& cc, u1d::ChainArgs* c, int* nblocks) {
+  const int nz = r.N[2], nt = r.N[3], bpp = r.bpp, per = r.bpp * (cc.block / 64), tz = cc.tile;
+  const std::string key = std::to_string(nz) + " " + std::to_string(nt) + " " + std::to_string(bpp) + " " +
+                          std::to_string(tz) + " " + std::to_string(cc.margin);
+  auto it = G.chain_sched.find(key);
+  if (it == G.chain_sched.end()) {
+    const u1d::ChainSchedule s = u1d::chain_schedule(nz, nt, bpp, tz, cc.margin);
+    uint32_t* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, s.list.size() * sizeof(uint32_t)));
+    const hipError_t e = hipMemcpy(d, s.list.data(), s.list.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(U1_ERR_HIP, std::string("chain schedule upload: ") + hipGetErrorString(e));
+    }
+    it = G.chain_sched.emplace(key, d).first;
+    G.chain_sched_len[key] = (int)s.list.size();
+  }
+  const size_t need = 4 * (size_t)nz * (size_t)nt;
+  bool clear = nz != G.chain_nz || nt != G.chain_nt || per != G.chain_per;
+  for (int mu = 0; mu < 4; ++mu) clear = clear || (uint64_t)(G.chain_inc[mu] + 2) * (uint64_t)per >= (1ull << 31);
+  if (need > G.chain_cnt_cap) {
+    HIPCHK(hipStreamSynchronize(G.st));  // no queued launch still uses the old counters
+    if (G.chain_cnt) HIPCHK(hipFree(G.chain_cnt));
+    G.chain_cnt = nullptr;
+    G.chain_cnt_cap = 0;
+    HIPCHK(hipMalloc((void**)&G.chain_cnt, need * sizeof(uint32_t)));
+    G.chain_cnt_cap = need;
+    clear = true;
+  }
+  if (!G.chain_err) {
+    HIPCHK(hipHostMalloc((void**)&G.chain_err, sizeof(uint32_t), hipHostMallocMapped));
+    *G.chain_err = 0;
+  }
+  if (clear) {
+    HIPCHK(u1d::launch_fill_zero(G.chain_cnt, (int64_t)(need * sizeof(uint32_t)), G.st));
+    G.chain_nz = nz;
+    G.chain_nt = nt;
+    G.chain_per = per;
+    for (int mu = 0; mu < 4; ++mu) G.chain_inc[mu] = 0;
+  }
+  void* err_dev = nullptr;
+  HIPCHK(hipHostGetDevicePointer(&err_dev, G.chain_err, 0));
+  c->sched = it->second;
+  c->cnt = G.chain_cnt;
+  c->err = (uint32_t*)err_dev;
+  for (int mu = 0; mu < 4; ++mu) c->tgt[mu] = (G.chain_inc[mu] + 1) * (uint32_t)per;
+  c->tz_log2 = tz == 8 ? 3 : tz == 4 ? 2 : tz == 2 ? 1 : 0;
+  c->bz = (uint32_t)(nz / tz);
+  c->bt = (uint32_t)(nt / (8 / tz));
+  const u1d::MagicDiv md = u1d::make_magic(c->bz);
+  c->bz_m = md.m;
+  c->bz_s = md.s;
+  *nblocks = 8 * G.chain_sched_len[key];
+  return U1_OK;
+}
+
+// One fused pass (phases (a.parity, 0..3)) in shape cc; a.meas as for
+// launch_chain.
+int chain_launch(int prec, int kind, void* lat, const PhaseArgs& a0, const Cfg& cc) {
+  PhaseArgs a = a0;
+  a.xcd = 0;
+  set_chain_blocks(a.single, cc);
+  u1d::ChainArgs c{};
+  int nb = 0;
+  RET(chain_prepare(a.single, cc, &c, &nb));
+  HIPCHK(u1d::launch_chain(prec, kind, lat, a, c, nb, cc.block, cc.spt, G.st));
+  for (int mu = 0; mu < 4; ++mu) ++G.chain_inc[mu];
+  return U1_OK;
+}
+
+// The four phases of one parity as the unfused path launches them: the
+// fused (parity, 0) + (parity, 1) pair if pc says so, then the phases.
+int unfused_parity_pass(int prec, int kind, void* lat, const PhaseArgs& a0, const Cfg& c, const Cfg& pc) {
+  PhaseArgs a = a0;
+  for (int mu = 0; mu < 4; ++mu) {
+    if (mu == 0 && pc.block != 0) {
+      a.single = a0.single;
+      a.xcd = (uint32_t)pc.xcd;
+      set_pair_blocks(a.single, pc);
+      HIPCHK(u1d::launch_pair(prec, kind, lat, a, a.single.nblk, pc.block, pc.spt, a.single.nxh * a.single.N[1], G.st));
+      ++mu;
+    } else {
+      a.single = a0.single;
+      a.xcd = (uint32_t)c.xcd;
+      set_update_blocks(a.single, c);
+      HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, a.single.nblk, c.block, c.spt, G.st));
+    }
+  }
+  return U1_OK;
+}
+
+// Shape of the fused parity pass for kernel class `kind` on this geometry, or
+// block = 0 for the unfused launches.  With tuning, every fused shape is timed
+// against the unfused pass (tuned phase and pair shapes c, pc) on a scratch
+// copy: four passes each (parities 0, 1, 0, 1); the decision is cached
+// (MetropolisChain / OverRelaxationChain, block.x 0 = unfused).  Only for an
+// unmasked single lattice whose (z, t) planes tile 2 x 4 (Nz even, Nt % 4 ==
+// 0), and never during stream capture (the counter targets are per launch).
+int chain_cfg(int prec, int kind, const u1_geom* g, const void* lat, const PhaseArgs& proto, const Cfg& c,
+              const Cfg& pc, Cfg* cfg) {
+  *cfg = Cfg{0, 1, 0, 0};
+  if (G.chain == 0 || proto.ml != 0) return U1_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(G.st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return U1_OK;
+  const std::vector<Cfg> shapes = chain_shapes(proto.single);
+  if (shapes.empty()) return U1_OK;
+  const char* name = kind == u1d::KIND_METROPOLIS ? "MetropolisChain" : "OverRelaxationChain";
+  const std::string key = tune_key(name, g, prec);
+  auto forced_default = [&]() {
+    *cfg = shapes[0];
+    for (const Cfg& k : shapes)
+      if (k.block == 256 && k.spt == 1) *cfg = k;
+  };
+  auto it = G.tunecache.find(key);
+  if (it != G.tunecache.end()) {  // block 0: the tuner measured the unfused launches faster
+    if (it->second.block != 0 && chain_shape_ok(proto.single, it->second)) *cfg = it->second;
+    else if (G.chain == 2) forced_default();
+    return U1_OK;
+  }
+  if (!G.tune) {
+    if (G.chain == 2) forced_default();
+    return U1_OK;
+  }
+  const int64_t bytes = 4 * (int64_t)proto.single.V * (int64_t)elem_size(prec);
+  DevBuf scratch;
+  HIPCHK(scratch.alloc(bytes));
+  HIPCHK(hipMemcpyAsync(scratch.p, lat, bytes, hipMemcpyDeviceToDevice, G.st));
+  EventSet es;
+  hipEvent_t e0, e1;
+  HIPCHK(es.create(&e0));
+  HIPCHK(es.create(&e1));
+  PhaseArgs a = proto;
+  a.accepts = nullptr;
+  a.meas = nullptr;
+  auto timed = [&](bool fused, const Cfg& cc, float* ms) -> int {
+    auto one = [&](int i) -> int {
+      a.parity = i & 1;
+      if (fused) return chain_launch(prec, kind, scratch.p, a, cc);
+      return unfused_parity_pass(prec, kind, scratch.p, a, c, pc);
+    };
+    RET(one(0));  // warm-up
+    HIPCHK(hipEventRecord(e0, G.st));
+    for (int i = 0; i < 4; ++i) RET(one(i));
+    HIPCHK(hipEventRecord(e1, G.st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    return chain_check();
+  };
+  // three rounds over the candidates, best time per candidate; the unfused
+  // pass timed in every round too
+  float tsep = 1e30f;
+  auto search = [&](const std::vector<Cfg>& cands, std::vector<float>& tbest) -> int {
+    tbest.assign(cands.size(), 1e30f);
+    for (int round = 0; round < 3; ++round) {
+      for (size_t i = 0; i < cands.size(); ++i) {
+        float ms = 0;
+        RET(timed(true, cands[i], &ms));
+        tbest[i] = std::min(tbest[i], ms);
+      }
+      float ms = 0;
+      RET(timed(false, Cfg{0, 1, 0, 0}, &ms));
+      tsep = std::min(tsep, ms);
+    }
+    return U1_OK;
+  };
+  auto report = [&](const std::vector<Cfg>& cands, const std::vector<float>& tbest, size_t ib) {
+    if (G.verbosity >= 2)
+      for (size_t i = 0; i < cands.size(); ++i)
+        std::fprintf(stderr, "tune %s: block %d, %d site(s)/thread, %d z tiles, margin %d: %.2f us per fused pass%s\n",
+                     key.c_str(), cands[i].block, cands[i].spt, cands[i].tile, cands[i].margin,
+                     1e3 * tbest[i] / 4.0, i == ib ? "  <- best" : "");
+  };
+  auto argmin = [](const std::vector<float>& t) {
+    size_t ib = 0;
+    for (size_t i = 1; i < t.size(); ++i)
+      if (t[i] < t[ib]) ib = i;
+    return ib;
+  };
+  std::vector<float> t1, t2;
+  RET(search(shapes, t1));
+  const size_t i1 = argmin(t1);
+  report(shapes, t1, i1);
+  const std::vector<Cfg> layouts = chain_layouts(proto.single, shapes[i1]);
+  RET(search(layouts, t2));
+  const size_t i2 = argmin(t2);
+  report(layouts, t2, i2);
+  const Cfg best = layouts[i2];
+  const float tb = t2[i2];
+  // chosen only on a clear win: the two paths' timings move by a few per cent
+  // from one tuning run to the next
+  const bool faster = tb < kChainMinGain * tsep;
+  const bool use = G.chain == 2 || faster;
+  if (G.verbosity >= 1)
+    std::fprintf(stderr,
+                 "tune %s: fused pass %.2f us (block %d, %d site(s)/thread, %d z tiles, margin %d) vs unfused %.2f "
+                 "us: %s\n",
+                 key.c_str(), 1e3 * tb / 4.0, best.block, best.spt, best.tile, best.margin, 1e3 * tsep / 4.0,
+                 use ? "fused" : "unfused");
+  G.tunecache[key] = faster ? best : Cfg{0, 1, 0, 0};
+  *cfg = use ? best : Cfg{0, 1, 0, 0};
+  return U1_OK;
+}
+
+// Settles (tunes, if enabled) every launch shape a sweep of this geometry
+// uses, and the fused passes' schedules and counters, so that a later sweep
+// allocates and synchronises nothing (graph capture, timed regions).
+int settle_update_cfgs(int prec, const u1_geom* g, const void* lat, const PhaseArgs& proto, int metrop, int ovrn) {
+  Cfg c, pc, cc;
+  for (int kind = 0; kind < 2; ++kind) {
+    if ((kind == u1d::KIND_METROPOLIS ? metrop : ovrn) <= 0) continue;
+    RET(phase_cfg(prec, kind, g, lat, proto, &c));
+    RET(pair_cfg(prec, kind, g, lat, proto, c, &pc));
+    RET(chain_cfg(prec, kind, g, lat, proto, c, pc, &cc));
+    if (cc.block != 0) {
+      RepDesc r = proto.single;
+      set_chain_blocks(r, cc);
+      u1d::ChainArgs ca{};
+      int nb = 0;
+      RET(chain_prepare(r, cc, &ca, &nb));
+    }
   }
   return U1_OK;
 }
@@ -517,11 +795,13 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   a.sweep_dev = sweep_dev;
   a.sweep_mul = sweep_mul;
   a.accepts = acc_dev;
-  Cfg cm = kDefaultCfg, co = kDefaultCfg, pm{0, 1, 0, 0}, po{0, 1, 0, 0};
+  Cfg cm = kDefaultCfg, co = kDefaultCfg, pm{0, 1, 0, 0}, po{0, 1, 0, 0}, chm{0, 1, 0, 0}, cho{0, 1, 0, 0};
   if (metrop > 0) RET(phase_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, &cm));
   if (ovrn > 0) RET(phase_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, &co));
   if (metrop > 0) RET(pair_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, cm, &pm));
   if (ovrn > 0) RET(pair_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, co, &po));
+  if (metrop > 0) RET(chain_cfg(prec, u1d::KIND_METROPOLIS, g, lat, a, cm, pm, &chm));
+  if (ovrn > 0) RET(chain_cfg(prec, u1d::KIND_OVERRELAX, g, lat, a, co, po, &cho));
   RepDesc rm = a.single, ro = a.single, rpm = a.single, rpo = a.single;
   set_update_blocks(rm, cm);
   set_update_blocks(ro, co);
@@ -529,22 +809,35 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
   set_pair_blocks(rpo, po);
   if (meas_dst && (ml != 0 || metrop + ovrn == 0)) return fail(U1_ERR_ARG, "no measuring pass");
   // measuring phases: parity 1, mu >= 1 of the last pass; rows of phase 1 (in
-  // a fused pair: one per plane), then of phases 2 and 3
+  // a fused pair: one per plane), then of phases 2 and 3 (a fused pass: its
+  // own rows for phases 1-3, launch_chain)
   const bool meas_or = ovrn > 0;
+  const Cfg& meas_chain = meas_or ? cho : chm;
   const int meas_nblk = meas_or ? ro.nblk : rm.nblk;
   const int meas_nblk1 = (meas_or ? po.block : pm.block) != 0 ? (meas_or ? rpo.nblk : rpm.nblk) : meas_nblk;
-  const int meas_rows_total = meas_nblk1 + 2 * meas_nblk;
+  int meas_rows_total = meas_nblk1 + 2 * meas_nblk;
+  if (meas_chain.block != 0) {
+    RepDesc rc = a.single;
+    set_chain_blocks(rc, meas_chain);
+    meas_rows_total = 3 * rc.N[2] * rc.N[3] * rc.bpp;
+  }
   if (meas_dst) RET(ensure_fold(meas_rows_total, 4));
   auto meas_rows = [&](bool last_pass, int parity, int mu) -> double* {
     if (!meas_dst || !last_pass || parity != 1 || mu < 1) return nullptr;
     return G.partials + 4 * (mu == 1 ? 0 : (size_t)meas_nblk1 + (size_t)meas_nblk * (size_t)(mu - 2));
   };
-  // one pass of kind `kind`: per parity, (parity, 0) + (parity, 1) fused or
-  // not, then (parity, 2) and (parity, 3)
+  // one pass of kind `kind`: per parity, one fused pass (chc), or (parity, 0)
+  // + (parity, 1) fused or not, then (parity, 2) and (parity, 3)
   auto pass = [&](int kind, uint32_t p, bool last, const Cfg& c, const RepDesc& r, const Cfg& pc,
-                  const RepDesc& rp) -> int {
+                  const RepDesc& rp, const Cfg& chc) -> int {
     a.pass = p;
-    for (int parity = 0; parity < 2; ++parity)
+    for (int parity = 0; parity < 2; ++parity) {
+      if (chc.block != 0) {
+        a.parity = parity;
+        a.meas = meas_rows(last, parity, 1);  // rows from 0: phases 1-3 of the fused pass
+        RET(chain_launch(prec, kind, lat, a, chc));
+        continue;
+      }
       for (int mu = 0; mu < 4; ++mu) {
         const bool pair = mu == 0 && pc.block != 0;
         a.parity = parity;
@@ -560,11 +853,12 @@ int run_update(const u1_geom* g, int prec, void* lat, int metrop, int ovrn, cons
           HIPCHK(u1d::launch_phase(prec, kind, mu, lat, a, r.nblk, c.block, c.spt, G.st));
         }
       }
+    }
     return U1_OK;
   };
   for (int p = 0; p < metrop; ++p)
-    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, ovrn == 0 && p == metrop - 1, cm, rm, pm, rpm));
-  for (int o = 0; o < ovrn; ++o) RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == ovrn - 1, co, ro, po, rpo));
+    RET(pass(u1d::KIND_METROPOLIS, (uint32_t)p, ovrn == 0 && p == metrop - 1, cm, rm, pm, rpm, chm));
+  for (int o = 0; o < ovrn; ++o) RET(pass(u1d::KIND_OVERRELAX, (uint32_t)o, o == ovrn - 1, co, ro, po, rpo, cho));
   if (meas_dst)
     HIPCHK(u1d::launch_finish_sums(G.partials, meas_rows_total, 4, 1.0 / (3.0 * (double)a.single.V), meas_dst,
                                    fold_scratch_ptr(meas_rows_total, 4), G.st));
@@ -604,7 +898,7 @@ int poly_volume_async(int prec, const void* lat, const RepDesc& r, int mode, con
 int copy_back(double* host, const double* dev, size_t n) {
   HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, G.st));
   HIPCHK(hipStreamSynchronize(G.st));
-  return U1_OK;
+  return chain_check();
 }
 
 }  // namespace
@@ -912,7 +1206,18 @@ int ensemble_measured_sweeps(u1_ensemble* e, int32_t nsweeps, double* series, hi
 }
 
 // Launch classes of a sweep, for kernel timing.
-enum LaunchClass { LC_METRO = 0, LC_OR, LC_OR_MEAS, LC_METRO_PAIR, LC_OR_PAIR, LC_OR_PAIR_MEAS, LC_COUNT };
+enum LaunchClass {
+  LC_METRO = 0,
+  LC_OR,
+  LC_OR_MEAS,
+  LC_METRO_PAIR,
+  LC_OR_PAIR,
+  LC_OR_PAIR_MEAS,
+  LC_METRO_CHAIN,   // fused parity passes (single lattices only)
+  LC_OR_CHAIN,
+  LC_OR_CHAIN_MEAS,
+  LC_COUNT
+};
 constexpr int kTimingLaunches = 32;  // back-to-back launches per class
 
 // Kernel timing of the ensemble's launch classes (shapes settled, ovrn >= 1):
@@ -944,7 +1249,7 @@ int time_ensemble_classes(u1_ensemble* e, double* ms, double* per_sweep) {
   const bool fused = e->pcfg.block != 0;
   for (int c = 0; c < LC_COUNT; ++c) ms[c] = per_sweep[c] = 0.0;
   for (int c = 0; c < LC_COUNT; ++c) {
-    if (c == LC_OR_PAIR_MEAS) continue;  // the measuring parity-1 pair stays unfused
+    if (c == LC_OR_PAIR_MEAS || c >= LC_METRO_CHAIN) continue;  // the measuring parity-1 pair stays unfused
     const bool pair = c == LC_METRO_PAIR || c == LC_OR_PAIR;
     const int kind = (c == LC_METRO || c == LC_METRO_PAIR) ? u1d::KIND_METROPOLIS : u1d::KIND_OVERRELAX;
     const int passes = kind == u1d::KIND_METROPOLIS ? e->metrop : e->ovrn;
@@ -1032,6 +1337,15 @@ int u1_finalize(void) {
   G.work_cap = 0;
   if (G.counter) HIPCHK(hipFree(G.counter));
   if (G.dres) HIPCHK(hipFree(G.dres));
+  for (auto& kv : G.chain_sched) HIPCHK(hipFree(kv.second));
+  G.chain_sched.clear();
+  G.chain_sched_len.clear();
+  if (G.chain_cnt) HIPCHK(hipFree(G.chain_cnt));
+  if (G.chain_err) HIPCHK(hipHostFree(G.chain_err));
+  G.chain_cnt = nullptr;
+  G.chain_cnt_cap = 0;
+  G.chain_err = nullptr;
+  G.chain_nz = G.chain_nt = G.chain_per = 0;
   HIPCHK(hipStreamDestroy(G.st));
   G.partials = nullptr;
   G.partials_cap = 0;
@@ -1045,7 +1359,7 @@ int u1_finalize(void) {
 int u1_sync(void) {
   REQ_INIT();
   HIPCHK(hipStreamSynchronize(G.st));
-  return U1_OK;
+  return chain_check();
 }
 
 const char* u1_last_error(void) { return g_err.c_str(); }
@@ -1133,6 +1447,7 @@ int u1_update(const u1_geom* g, int prec, void* lat, int32_t metrop, int32_t ovr
     unsigned long long v = 0;
     HIPCHK(hipMemcpyAsync(&v, G.counter, sizeof(v), hipMemcpyDeviceToHost, G.st));
     HIPCHK(hipStreamSynchronize(G.st));
+    RET(chain_check());
     *accepts += v;
   }
   return U1_OK;
@@ -1737,6 +2052,12 @@ int u1_set_pair_fusion(int32_t mode) {
   return U1_OK;
 }
 
+int u1_set_pass_fusion(int32_t mode) {
+  if (mode < 0 || mode > 2) return fail(U1_ERR_ARG, "pass fusion mode must be 0, 1 or 2");
+  G.chain = mode;
+  return U1_OK;
+}
+
 int u1_set_verbosity(int32_t level) {
   if (level < 0 || level > 3) return fail(U1_ERR_ARG, "verbosity must be in [0, 3]");
   G.verbosity = level;
@@ -1776,6 +2097,14 @@ int u1_tune_load(const char* path) {
     // whole wavefronts, at most 512 threads and 4 sites per thread (binary32)
     const bool f32 = name.size() > 4 && name.compare(name.size() - 4, 4, "_f32") == 0;
     bool known = false;
+    if (name.find("Chain_") != std::string::npos) {
+      // fused pass: block 0 = unfused, block.z = tiles along z, grid.x = schedule margin
+      if (!((bx == 0 && by == 1) || ((by == 1 || by == 2) && bx >= 64 && bx <= 512 && bx % 64 == 0 &&
+                                     (bz == 1 || bz == 2 || bz == 4 || bz == 8) && gx >= 0 && gx <= kChainMaxMargin)))
+        return fail(U1_ERR_IO, std::string("Bad launch shape in ") + path + ": " + line);
+      parsed.push_back({vol + "\t" + name, Cfg{bx, by, 1, bz, bx == 0 ? 0 : gx}});
+      continue;
+    }
     if (name.find("Pair_") != std::string::npos)  // fused pair: block 0 = two launches
       known = (bx == 0 && by == 1) || pair_shape_ok(f32 ? U1_F32 : U1_F64, Cfg{bx, by, 0, 0});
     else
@@ -1799,10 +2128,13 @@ int u1_tune_save(const char* path) {
     const std::string& key = kv.first;
     const size_t tab = key.find('\t');
     const std::string vol = key.substr(0, tab), name = key.substr(tab + 1);
+    const bool chain = name.find("Chain_") != std::string::npos;
     f << vol << "\t" << name << "\tthreads=halfvolume\t" << kv.second.block << "\t" << kv.second.spt << "\t"
-      << 1 + kv.second.xcd + 2 * kv.second.tile
-      << "\t0\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z - 1 = XCD-aware order (1) + "
-         "tiled planes (2); Pair: fused (parity, 0) + (parity, 1) launch, block.x 0 = two launches\n";
+      << (chain ? kv.second.tile : 1 + kv.second.xcd + 2 * kv.second.tile) << "\t" << (chain ? kv.second.margin : 0)
+      << "\t1\t1\t0\t# block.x = workgroup size, block.y = sites per thread, block.z - 1 = XCD-aware order (1) + "
+         "tiled planes (2); Pair: fused (parity, 0) + (parity, 1) launch, block.x 0 = two launches; Chain: fused "
+         "parity pass (phases 0-3) in one launch, block.x 0 = unfused, block.z = tiles along z, grid.x = schedule "
+         "margin\n";
   }
   return f ? U1_OK : fail(U1_ERR_IO, "write failed");
 }
@@ -1829,12 +2161,14 @@ int time_launch_classes(const u1_geom* g, int prec, const void* lat, int metrop,
   a.step = m->step;
   a.stream = U1_STREAM_UPDATE;
   a.sweep = sweep;
-  Cfg cfg[2] = {kDefaultCfg, kDefaultCfg}, pcfg[2] = {{0, 1, 0, 0}, {0, 1, 0, 0}};
+  Cfg cfg[2] = {kDefaultCfg, kDefaultCfg}, pcfg[2] = {{0, 1, 0, 0}, {0, 1, 0, 0}},
+      ccfg[2] = {{0, 1, 0, 0}, {0, 1, 0, 0}};
   const int passes[2] = {metrop, ovrn};
   for (int kind = 0; kind < 2; ++kind)
     if (passes[kind] > 0) {
       RET(phase_cfg(prec, kind, g, lat, a, &cfg[kind]));
       RET(pair_cfg(prec, kind, g, lat, a, cfg[kind], &pcfg[kind]));
+      RET(chain_cfg(prec, kind, g, lat, a, cfg[kind], pcfg[kind], &ccfg[kind]));
     }
   const int64_t bytes = 4 * (int64_t)a.single.V * (int64_t)elem_size(prec);
   DevBuf scratch;
@@ -1849,12 +2183,42 @@ int time_launch_classes(const u1_geom* g, int prec, const void* lat, int metrop,
   // launches of mu >= 1 (the parity-1 pair, when fused)
   const bool meas_or = meas && ovrn > 0;
   for (int c = 0; c < LC_COUNT; ++c) {
-    const bool pair = c >= LC_METRO_PAIR;
-    const int kind = (c == LC_METRO || c == LC_METRO_PAIR) ? u1d::KIND_METROPOLIS : u1d::KIND_OVERRELAX;
-    const bool cmeas = c == LC_OR_MEAS || c == LC_OR_PAIR_MEAS;
+    const bool chain = c >= LC_METRO_CHAIN;
+    const bool pair = c >= LC_METRO_PAIR && !chain;
+    const int kind = (c == LC_METRO || c == LC_METRO_PAIR || c == LC_METRO_CHAIN) ? u1d::KIND_METROPOLIS
+                                                                                  : u1d::KIND_OVERRELAX;
+    const bool cmeas = c == LC_OR_MEAS || c == LC_OR_PAIR_MEAS || c == LC_OR_CHAIN_MEAS;
     const Cfg& pc = pcfg[kind];
     const bool fused = pc.block != 0;
-    if (passes[kind] == 0 || (pair && !fused) || (cmeas && !meas_or)) continue;
+    const bool chained = ccfg[kind].block != 0;
+    if (passes[kind] == 0 || (cmeas && !meas_or) || chain != chained) continue;
+    if (chain) {
+      // one launch per parity and pass; the parity-1 launch of the last pass measures
+      per_sweep[c] = cmeas ? 1.0 : 2.0 * passes[kind] - (meas_or && kind == u1d::KIND_OVERRELAX ? 1 : 0);
+      if (cmeas) {
+        RepDesc rc = a.single;
+        set_chain_blocks(rc, ccfg[kind]);
+        RET(ensure_fold(3 * rc.N[2] * rc.N[3] * rc.bpp, 4));
+      }
+      PhaseArgs b = a;
+      auto launch = [&](int i) -> int {
+        b.parity = cmeas ? 1 : i & 1;
+        b.pass = (uint32_t)(i >> 1);
+        b.meas = cmeas ? G.partials : nullptr;
+        return chain_launch(prec, kind, scratch.p, b, ccfg[kind]);
+      };
+      RET(launch(0));  // warm-up
+      HIPCHK(hipEventRecord(e0, G.st));
+      for (int i = 0; i < kTimingLaunches; ++i) RET(launch(i));
+      HIPCHK(hipEventRecord(e1, G.st));
+      HIPCHK(hipEventSynchronize(e1));
+      RET(chain_check());
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      ms[c] = t / kTimingLaunches;
+      continue;
+    }
+    if (pair && !fused) continue;
     // the (parity, mu) launches of one pass in this class (a pair is listed by mu 0)
     std::vector<std::pair<int, int>> ph;
     for (int parity = 0; parity < 2; ++parity)
@@ -1937,6 +2301,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
   }
   HIPCHK(hipEventRecord(e1, G.st));
   HIPCHK(hipEventSynchronize(e1));
+  RET(chain_check());
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *elapsed_ms = ms;

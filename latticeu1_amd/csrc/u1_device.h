@@ -109,6 +109,20 @@ template <typename T>
 __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
   *(T*)((char*)base + idx * (uint32_t)sizeof(T)) = v;
 }
+// The same at device scope (relaxed atomics: global_load / global_store with
+// the sc1 bit): the store is written through to the device's point of
+// coherence, and the load is served from there, not from a line that this
+// CU's or this XCD's cache filled earlier.  Used by the fused parity pass for
+// the angles one workgroup writes and another reads within the launch.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* base, uint32_t idx) {
+  return __hip_atomic_load((T*)((const char*)base + idx * (uint32_t)sizeof(T)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void stc(T* base, uint32_t idx, T v) {
+  __hip_atomic_store((T*)((char*)base + idx * (uint32_t)sizeof(T)), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Update-launch decomposition: workgroup b (relative to the replica's first)
 // covers chunk (b % bpp) of plane b / bpp in launch order, which is the
@@ -123,6 +137,31 @@ struct PhaseSite {
   int32_t z;
   int32_t fwd[4], bwd[4];
 };
+
+// Site i_raw (in-plane index; past the end: the plane's last site, invalid)
+// of plane (z, t).
+__host__ __device__ __forceinline__ void phase_site_zt(const RepDesc& r, int32_t z, uint32_t t, uint32_t i_raw,
+                                                       int32_t parity, PhaseSite& s) {
+  const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
+  s.valid = i_raw < P;
+  // a thread past the end of the plane takes the plane's last site: its loads
+  // stay in bounds (and hit the cache lines of that site); it stores nothing
+  const uint32_t i = s.valid ? i_raw : P - 1;
+  s.t = t;
+  s.z = z;
+  const uint32_t y = mdiv(i, r.div_nxh_m, r.div_nxh_s);
+  const int32_t xh = (int32_t)(i - y * (uint32_t)r.nxh);
+  s.id = ((uint32_t)z + (uint32_t)r.N[2] * t) * P + i;  // lexicographic plane z + Nz t
+  const int32_t xodd = ((int32_t)y + z + (int32_t)t + parity) & 1;
+  s.fwd[0] = xodd ? ((xh == r.nxh - 1) ? -(r.nxh - 1) : 1) : 0;
+  s.bwd[0] = xodd ? 0 : ((xh == 0) ? (r.nxh - 1) : -1);
+  s.fwd[1] = ((int32_t)y == r.N[1] - 1) ? -(r.N[1] - 1) * r.sy : r.sy;
+  s.bwd[1] = (y == 0) ? (r.N[1] - 1) * r.sy : -r.sy;
+  s.fwd[2] = (z == r.N[2] - 1) ? -(r.N[2] - 1) * r.sz : r.sz;
+  s.bwd[2] = (z == 0) ? (r.N[2] - 1) * r.sz : -r.sz;
+  s.fwd[3] = ((int32_t)t == r.N[3] - 1) ? -(r.N[3] - 1) * r.st : r.st;
+  s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
+}
 
 __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b, uint32_t tid, uint32_t bdim,
                                                     int32_t parity, PhaseSite& s) {
@@ -143,26 +182,7 @@ __host__ __device__ __forceinline__ void phase_site(const RepDesc& r, uint32_t b
     t = mdiv(plane, r.div_nz_m, r.div_nz_s);
     z = (int32_t)(plane - t * (uint32_t)r.N[2]);
   }
-  const uint32_t P = (uint32_t)r.nxh * (uint32_t)r.N[1];
-  const uint32_t i_raw = chunk * bdim + tid;
-  s.valid = i_raw < P;
-  // a thread past the end of the plane takes the plane's last site: its loads
-  // stay in bounds (and hit the cache lines of that site); it stores nothing
-  const uint32_t i = s.valid ? i_raw : P - 1;
-  s.t = t;
-  s.z = z;
-  const uint32_t y = mdiv(i, r.div_nxh_m, r.div_nxh_s);
-  const int32_t xh = (int32_t)(i - y * (uint32_t)r.nxh);
-  s.id = ((uint32_t)z + (uint32_t)r.N[2] * t) * P + i;  // lexicographic plane z + Nz t
-  const int32_t xodd = ((int32_t)y + z + (int32_t)t + parity) & 1;
-  s.fwd[0] = xodd ? ((xh == r.nxh - 1) ? -(r.nxh - 1) : 1) : 0;
-  s.bwd[0] = xodd ? 0 : ((xh == 0) ? (r.nxh - 1) : -1);
-  s.fwd[1] = ((int32_t)y == r.N[1] - 1) ? -(r.N[1] - 1) * r.sy : r.sy;
-  s.bwd[1] = (y == 0) ? (r.N[1] - 1) * r.sy : -r.sy;
-  s.fwd[2] = (z == r.N[2] - 1) ? -(r.N[2] - 1) * r.sz : r.sz;
-  s.bwd[2] = (z == 0) ? (r.N[2] - 1) * r.sz : -r.sz;
-  s.fwd[3] = ((int32_t)t == r.N[3] - 1) ? -(r.N[3] - 1) * r.st : r.st;
-  s.bwd[3] = (t == 0) ? (r.N[3] - 1) * r.st : -r.st;
+  phase_site_zt(r, z, t, chunk * bdim + tid, parity, s);
 }
 
 // XCD-aware workgroup order.  The dispatcher hands workgroup b of a launch to

@@ -53,6 +53,27 @@ constexpr int pair_max_plane(int elem_bytes) { return 65536 / elem_bytes; }
 hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, int nblocks, int block, int spt,
                        int plane_sites, hipStream_t st);
 
+// The four phases (parity, 0..3) of one pass in one launch (k_chain, schedule
+// and dependencies in u1_chain.h).  A single lattice without multilevel mask:
+// a.single carries set_phase_blocks(r, block * spt); sched is the list of
+// chain_schedule(Nz, Nt, bpp, tz) in device memory, nblocks = 8 * its length,
+// and (tz_log2, bz, bt, bz_m, bz_s) its tiling: 2^tz_log2 tiles of bz planes
+// along z, 8 / 2^tz_log2 of bt along t, magic division by bz.  cnt holds 4 Nz Nt completion counters (row mu,
+// plane z + Nz t); every wavefront adds 1 to its (mu, plane) counter when
+// done, and phase mu waits until the counters of row mu - 1 it depends on
+// reach tgt[mu - 1].  A wait that times out sets *err (host-visible memory).
+// a.meas makes phases 1-3 measuring: row (mu - 1) Nz Nt bpp + (z + Nz t) bpp
+// + chunk.  spt 1 or 2, block a multiple of 64 up to 512.
+struct ChainArgs {
+  const uint32_t* sched;
+  uint32_t* cnt;
+  uint32_t* err;
+  uint32_t tgt[4];
+  uint32_t tz_log2, bz, bt, bz_m, bz_s;
+};
+hipError_t launch_chain(int prec, int kind, void* lat_base, const PhaseArgs& a, const ChainArgs& c, int nblocks,
+                        int block, int spt, hipStream_t st);
+
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t st);
 
 // measurement launchers (u1_measure.hip)

@@ -26,6 +26,7 @@
 // v_cos_f32, v_exp_f32, v_rcp_f32) and explicit FMAs; the periodic wrap into
 // [-pi, pi] in binary64 (wrap_near_d), rounded once to binary32.  It reads the
 // same Philox words as the FP64 path.  Its parity bar is statistical.
+#include "u1_chain.h"
 #include "u1_kernels.h"
 
 namespace u1d {
@@ -64,9 +65,10 @@ struct StapleIn {
   Real a[3], b[3], c[3], d[3], e[3], f[3];
 };
 
-// Loads the staple angles.  With DEFER0 the two of them that read theta_0 at
-// the link's own parity (c and d of nu = 0) are left to phase_load_nu0_lds.
-template <typename Real, int MU, bool DEFER0 = false>
+// Loads the staple angles.  The c and d angles of the directions nu in the
+// bit mask DEFM (theta_nu at the link's own parity) are left out: for a fused
+// pair (DEFM = 1) to phase_load_nu0_lds, for a fused pass to phase_load_dep.
+template <typename Real, int MU, int DEFM = 0>
 __device__ __forceinline__ void staple_load(const Real* __restrict__ lat, const RepDesc& r, uint32_t same,
                                             uint32_t opp, const int32_t (&fwd)[4], const int32_t (&bwd)[4],
                                             StapleIn<Real>& in) {
@@ -78,7 +80,7 @@ __device__ __forceinline__ void staple_load(const Real* __restrict__ lat, const 
     const Real* pn = lat + (size_t)nu * r.V;
     in.a[j] = ld(pn, opp + fwd[MU]);
     in.b[j] = ld(pm, opp + fwd[nu]);
-    if (!(DEFER0 && nu == 0)) {
+    if (!((DEFM >> nu) & 1)) {
       in.c[j] = ld(pn, same);
       in.d[j] = ld(pn, same + fwd[MU] + bwd[nu]);
     }
@@ -230,8 +232,8 @@ __device__ __forceinline__ bool phase_sites(const PhaseArgs& a, const RepDesc& r
   return MU != 3 && a.ml != 0 && (ps[0].t & (uint32_t)(a.ml - 1)) == 0;
 }
 
-// The angle loads of the sites in q.ps.
-template <typename Real, int MU, int SPT, bool DEFER0 = false>
+// The angle loads of the sites in q.ps (but the c, d angles of DEFM).
+template <typename Real, int MU, int SPT, int DEFM = 0>
 __device__ __forceinline__ void phase_load(const Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
                                            PhaseIn<Real, SPT>& q) {
   const Real* lat = lat_base + r.lat_off;
@@ -239,12 +241,31 @@ __device__ __forceinline__ void phase_load(const Real* __restrict__ lat_base, co
   for (int k = 0; k < SPT; ++k) {
     q.same[k] = (uint32_t)(a.parity * r.HV) + q.ps[k].id;
     const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + q.ps[k].id;
-    staple_load<Real, MU, DEFER0>(lat, r, q.same[k], opp, q.ps[k].fwd, q.ps[k].bwd, q.in[k]);
+    staple_load<Real, MU, DEFM>(lat, r, q.same[k], opp, q.ps[k].fwd, q.ps[k].bwd, q.in[k]);
     q.th[k] = ld(lat + (size_t)MU * r.V, q.same[k]);
   }
 }
 
-// The two angles phase_load<..., DEFER0 = true> left out -- theta_0 at the
+// The c, d angles of the directions in DEFM that phase_load left out --
+// theta_nu(x) and theta_nu(x + mu - nu) at the link's own parity, written
+// earlier in the same launch by another workgroup -- as device-scope loads.
+template <typename Real, int MU, int SPT, int DEFM>
+__device__ __forceinline__ void phase_load_dep(const Real* __restrict__ lat_base, const RepDesc& r,
+                                               PhaseIn<Real, SPT>& q) {
+  const Real* lat = lat_base + r.lat_off;
+#pragma unroll
+  for (int k = 0; k < SPT; ++k)
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+      if (nu == MU || !((DEFM >> nu) & 1)) continue;
+      const int j = nu < MU ? nu : nu - 1;
+      const Real* pn = lat + (size_t)nu * r.V;
+      q.in[k].c[j] = ldc(pn, q.same[k]);
+      q.in[k].d[j] = ldc(pn, q.same[k] + (uint32_t)(q.ps[k].fwd[MU] + q.ps[k].bwd[nu]));
+    }
+}
+
+// The two angles phase_load<..., DEFM = 1> left out -- theta_0 at the
 // link's own parity at x and x + e1 - e0, both in the site's (z, t) plane --
 // from lds_th, where phase 0 of this workgroup left the plane's new theta_0
 // by in-plane index (id - pbase).
@@ -259,8 +280,9 @@ __device__ __forceinline__ void phase_load_nu0_lds(const Real* lds_th, uint32_t 
 }
 
 // lds_th (nullable): the new angles are also written to lds_th[id - pbase],
-// the site's index within its (z, t) plane (fused pairs, phase 0).
-template <typename Real, int MU, int KIND, int SPT, bool MEAS>
+// the site's index within its (z, t) plane (fused pairs, phase 0).  COH: the
+// new angles are stored at device scope (fused passes).
+template <typename Real, int MU, int KIND, int SPT, bool MEAS, bool COH = false>
 __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
                                               PhaseIn<Real, SPT>& q, double (&v)[2], uint32_t& acc_wave,
                                               Real* lds_th = nullptr, uint32_t pbase = 0) {
@@ -340,7 +362,10 @@ __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const
 #pragma unroll
   for (int k = 0; k < SPT; ++k)
     if (ps[k].valid) {
-      st(pmu, same[k], th[k]);
+      if (COH)
+        stc(pmu, same[k], th[k]);
+      else
+        st(pmu, same[k], th[k]);
       if (lds_th) lds_th[ps[k].id - pbase] = th[k];
     }
   if (MEAS) {
@@ -438,7 +463,7 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
 #pragma unroll
     for (int k = 0; k < SPT; ++k) q1.ps[k] = q0.ps[k];  // the same sites
     phase_load<Real, 0, SPT>(lat_base, a, r, q0);
-    phase_load<Real, 1, SPT, true>(lat_base, a, r, q1);
+    phase_load<Real, 1, SPT, 1>(lat_base, a, r, q1);
     phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q0, v0, acc0, th0, pbase);
     __syncthreads();
     phase_load_nu0_lds<Real, SPT>(th0, pbase, q1);
@@ -456,12 +481,108 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
       phase_sites<1, SPT>(a, r, b, base + threadIdx.x, step, q.ps);
-      phase_load<Real, 1, SPT, true>(lat_base, a, r, q);
+      phase_load<Real, 1, SPT, 1>(lat_base, a, r, q);
       phase_load_nu0_lds<Real, SPT>(th0, pbase, q);
       phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
     }
   }
   phase_finish<1, KIND, MEAS>(a, L, v, acc0 + acc1);
+}
+
+// Reads a completion counter at device scope (sees other XCDs' increments);
+// the value is wave-uniform.
+__device__ __forceinline__ uint32_t chain_poll(const uint32_t* p) {
+  return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Waits (one wavefront) until the completion counter *p reaches target, with
+// a sleep between polls; after kChainSpin polls it records a timeout in *err
+// and returns.
+__device__ __forceinline__ void chain_wait(const uint32_t* p, uint32_t target, uint32_t* err) {
+  for (uint32_t n = 0; chain_poll(p) < target; ++n) {
+    if (n == kChainSpin) {
+      if ((threadIdx.x & 63) == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+// One phase-MU workgroup of a fused pass (k_chain): chunk `chunk` of plane
+// (z, t).  Each wavefront on its own:
+//  1. (MU > 0) reads the counters of the phase-(MU - 1) planes of chain_deps
+//     once, before its other loads, so that this read overlaps them;
+//  2. issues the loads that no workgroup of this launch writes: the angles at
+//     the other parity, the link's own angle, and theta_nu (nu > MU) at this
+//     parity, which phase nu only overwrites after this wavefront is counted;
+//  3. (MU > 0) waits until those counters are complete, if the first read did
+//     not find them so;
+//  4. loads theta_nu (nu < MU) at this parity at device scope;
+//  5. updates its links and stores them at device scope (plus the
+//     measurement row, the accept count);
+//  6. waits for its stores to complete (s_waitcnt vmcnt(0)) and adds 1 to
+//     the (MU, plane) counter at device scope.
+// A (mu, plane) counter is complete when every wavefront of the plane's bpp
+// workgroups has added to it (tgt: the launch's value, bpp * blockDim.x / 64
+// above the previous launch's).  Device-scope stores and loads make the
+// exchange coherent across the XCDs' L2 caches without the whole-cache
+// write-back and invalidation of an agent-scope fence.
+template <typename Real, int MU, int KIND, int SPT, bool MEAS>
+__device__ __forceinline__ void chain_phase(Real* __restrict__ lat_base, const PhaseArgs& a, const ChainArgs& c,
+                                            int32_t z, uint32_t t, uint32_t chunk) {
+  const RepDesc& r = a.single;
+  const uint32_t npl = (uint32_t)r.N[2] * (uint32_t)r.N[3];
+  const uint32_t plane = (uint32_t)z + (uint32_t)r.N[2] * t;
+  constexpr int DEFM = (1 << MU) - 1;
+  PhaseIn<Real, SPT> q;
+#pragma unroll
+  for (int k = 0; k < SPT; ++k)
+    phase_site_zt(r, z, t, chunk * (SPT * blockDim.x) + threadIdx.x + k * blockDim.x, a.parity, q.ps[k]);
+  const uint32_t* dep[2] = {nullptr, nullptr};
+  uint32_t seen[2] = {0, 0};
+  int ndep = 0;
+  if (MU > 0) {
+    int dz[2], dt[2];
+    ndep = chain_deps(MU, z, (int)t, r.N[2], r.N[3], dz, dt);
+    for (int i = 0; i < ndep; ++i) {
+      dep[i] = c.cnt + (size_t)(MU - 1) * npl + (uint32_t)dz[i] + (uint32_t)r.N[2] * (uint32_t)dt[i];
+      seen[i] = chain_poll(dep[i]);
+    }
+  }
+  phase_load<Real, MU, SPT, DEFM>(lat_base, a, r, q);
+  if (MU > 0) {
+    for (int i = 0; i < ndep; ++i)
+      if (seen[i] < c.tgt[MU - 1]) chain_wait(dep[i], c.tgt[MU - 1], c.err);
+    phase_load_dep<Real, MU, SPT, DEFM>(lat_base, r, q);
+  }
+  double v[2] = {0.0, 0.0};
+  uint32_t acc = 0;
+  phase_compute<Real, MU, KIND, SPT, MEAS, true>(lat_base, a, r, q, v, acc);
+  phase_finish<MU, KIND, MEAS>(a, (uint32_t)((MU > 0 ? MU - 1 : 0) * npl + plane) * (uint32_t)r.bpp + chunk, v, acc);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wavefront's stores have completed
+  if ((threadIdx.x & 63) == 0)
+    __hip_atomic_fetch_add(c.cnt + (size_t)MU * npl + plane, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The phases (parity, 0..3) of one pass of a single lattice in one launch:
+// workgroup b runs entry b >> 3 of the schedule list (u1_chain.h) on the
+// tile of XCD b & 7.  Results are those of four separate launches, bit for
+// bit: each link update does the same arithmetic on the same angles.  MEAS
+// makes phases 1-3 measuring.
+template <typename Real, int KIND, int SPT, bool MEAS>
+__global__ __launch_bounds__(512) void k_chain(Real* __restrict__ lat_base, PhaseArgs a, ChainArgs c) {
+  const uint32_t x = blockIdx.x & 7u;
+  const uint32_t e = c.sched[blockIdx.x >> 3];
+  const uint32_t w = e >> 10, chunk = (e >> 2) & 0xFFu;
+  const uint32_t wt = mdiv(w, c.bz_m, c.bz_s);
+  const uint32_t t = (x >> c.tz_log2) * c.bt + wt;
+  const int32_t z = (int32_t)((x & ((1u << c.tz_log2) - 1u)) * c.bz + (w - wt * c.bz));
+  switch (e & 3u) {
+    case 0: chain_phase<Real, 0, KIND, SPT, false>(lat_base, a, c, z, t, chunk); break;
+    case 1: chain_phase<Real, 1, KIND, SPT, MEAS>(lat_base, a, c, z, t, chunk); break;
+    case 2: chain_phase<Real, 2, KIND, SPT, MEAS>(lat_base, a, c, z, t, chunk); break;
+    default: chain_phase<Real, 3, KIND, SPT, MEAS>(lat_base, a, c, z, t, chunk); break;
+  }
 }
 
 template <typename Real>
@@ -554,6 +675,26 @@ hipError_t launch_pair_prec(int kind, Real* lat, const PhaseArgs& a, int nblocks
   return launch_pair_spt<Real, KIND_OVERRELAX>(spt, lat, a, nblocks, block, lds, s);
 }
 
+template <typename Real, int KIND, int SPT>
+hipError_t launch_chain_meas(Real* lat, const PhaseArgs& a, const ChainArgs& c, int nblocks, int block,
+                             hipStream_t s) {
+  if (a.meas != nullptr)
+    k_chain<Real, KIND, SPT, true><<<nblocks, block, 0, s>>>(lat, a, c);
+  else
+    k_chain<Real, KIND, SPT, false><<<nblocks, block, 0, s>>>(lat, a, c);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_chain_prec(int kind, Real* lat, const PhaseArgs& a, const ChainArgs& c, int nblocks, int block,
+                             int spt, hipStream_t s) {
+  if (kind == KIND_METROPOLIS)
+    return spt == 2 ? launch_chain_meas<Real, KIND_METROPOLIS, 2>(lat, a, c, nblocks, block, s)
+                    : launch_chain_meas<Real, KIND_METROPOLIS, 1>(lat, a, c, nblocks, block, s);
+  return spt == 2 ? launch_chain_meas<Real, KIND_OVERRELAX, 2>(lat, a, c, nblocks, block, s)
+                  : launch_chain_meas<Real, KIND_OVERRELAX, 1>(lat, a, c, nblocks, block, s);
+}
+
 }  // namespace
 
 hipError_t launch_phase(int prec, int kind, int mu, void* lat_base, const PhaseArgs& a, int nblocks, int block,
@@ -580,6 +721,23 @@ hipError_t launch_pair(int prec, int kind, void* lat_base, const PhaseArgs& a, i
   const size_t lds = (size_t)plane_sites * (size_t)elem;
   if (prec == PREC_F64) return launch_pair_prec<double>(kind, (double*)lat_base, a, nblocks, block, spt, lds, s);
   return launch_pair_prec<float>(kind, (float*)lat_base, with_f32_constants(a), nblocks, block, spt, lds, s);
+}
+
+hipError_t launch_chain(int prec, int kind, void* lat_base, const PhaseArgs& a, const ChainArgs& c, int nblocks,
+                        int block, int spt, hipStream_t s) {
+  const RepDesc& r = a.single;
+  const int64_t P = (int64_t)r.nxh * r.N[1], chunk = (int64_t)block * spt;
+  // a single unmasked lattice in a tz x (8 / tz) tiling, its planes cut into
+  // bpp chunks of block * spt sites, whole wavefronts within the launch bound
+  const int tz = c.tz_log2 <= 3 ? 1 << c.tz_log2 : 0;
+  if (!(spt == 1 || spt == 2) || block < 64 || block > 512 || block % 64 != 0 || a.reps != nullptr || a.ml != 0 ||
+      !chain_possible(r.N[2], r.N[3], r.bpp, tz) || c.bz * (uint32_t)tz != (uint32_t)r.N[2] ||
+      c.bt * (uint32_t)(8 / tz) != (uint32_t)r.N[3] || (int64_t)r.bpp * chunk < P ||
+      (int64_t)(r.bpp - 1) * chunk >= P || nblocks <= 0 || nblocks % 8 != 0 || c.sched == nullptr ||
+      c.cnt == nullptr || c.err == nullptr)
+    return hipErrorInvalidValue;
+  if (prec == PREC_F64) return launch_chain_prec<double>(kind, (double*)lat_base, a, c, nblocks, block, spt, s);
+  return launch_chain_prec<float>(kind, (float*)lat_base, with_f32_constants(a), c, nblocks, block, spt, s);
 }
 
 hipError_t launch_hotstart(int prec, void* lat, const RepDesc& r, int block, hipStream_t s) {

@@ -42,3 +42,22 @@ def test_fma_proposal_matches_two_step_form(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "proposal ok" in out.stdout and "wrap ok" in out.stdout
+
+
+def test_fused_pass_schedule_and_dependencies(tmp_path):
+    """The fused parity pass (u1_chain.h): every (phase, plane, chunk) once in
+    the per-XCD lists, every dependency strictly earlier in schedule time, and
+    an emulation of the launch under random interleavings (reads of unwritten
+    angles at start, the waits of chain_deps, reads of the new angles, writes)
+    that reproduces four sequential phases exactly and catches a dropped
+    dependency (tests/cpu_checks/check_chain.cpp)."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "check_chain"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "latticeu1_amd", "csrc"),
+                    os.path.join(HERE, "cpu_checks", "check_chain.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "schedule ok" in out.stdout and "emulation ok" in out.stdout

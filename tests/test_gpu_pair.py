@@ -18,8 +18,10 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture
 def forced(u1):
     u1.set_pair_fusion(2)
+    u1.set_pass_fusion(0)  # the pairs, not the fused parity passes that would replace them
     yield
     u1.set_pair_fusion(1)
+    u1.set_pass_fusion(1)
     u1.tune_enable(True)
 
 
@@ -85,6 +87,7 @@ def test_fp32_pair_matches_two_launches(u1, dims, tune):
     out = {}
     try:
         u1.tune_enable(tune)
+        u1.set_pass_fusion(0)
         for mode in (0, 2):
             u1.set_pair_fusion(mode)
             lat = u1.Lattice.cold(g, u1.F32)
@@ -94,6 +97,7 @@ def test_fp32_pair_matches_two_launches(u1, dims, tune):
             out[mode] = (lat.to_host(), acc, p)
     finally:
         u1.set_pair_fusion(1)
+        u1.set_pass_fusion(1)
         u1.tune_enable(True)
     say(f"fp32 pair vs two launches {dims} tune={tune}: accepts {out[2][1]}")
     assert np.array_equal(out[0][0], out[2][0])
@@ -120,6 +124,7 @@ def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt, bz):
     out = {}
     try:
         u1.tune_enable(False)
+        u1.set_pass_fusion(0)
         u1.tune_load(str(cache))
         for mode in (0, 2):
             u1.set_pair_fusion(mode)
@@ -130,6 +135,7 @@ def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt, bz):
             out[mode] = (lat.to_host(), acc, p)
     finally:
         u1.set_pair_fusion(1)
+        u1.set_pass_fusion(1)
         u1.tune_enable(True)
     say(f"fp32 pair shape {block} x {spt} (block.z {bz}) vs two launches: accepts {out[2][1]}")
     assert np.array_equal(out[0][0], out[2][0])
@@ -154,21 +160,27 @@ def test_kernel_timing_classes(u1):
     try:
         u1.tune_enable(False)
         got = {}
-        for mode in (0, 2):
-            u1.set_pair_fusion(mode)
+        for mode, pair, chain in (("plain", 0, 0), ("pairs", 2, 0), ("passes", 0, 2)):
+            u1.set_pair_fusion(pair)
+            u1.set_pass_fusion(chain)
             lat = u1.Lattice.cold(g, u1.F32)
             lat.hotstart(3)
             ms, km, series = lat.bench(1, 3, u1.BENCH_METRO, 3, 0, 12, 1, kernel_timing=True)
             got[mode] = (ms, km, series)
-            assert km[13] == 32 and ms > 0 and abs(km[12] - ms / 12) <= 1e-6 * ms
-            assert all(km[c] > 0 for c in range(6) if km[6 + c] > 0)
+            assert km[19] == 32 and ms > 0 and abs(km[18] - ms / 12) <= 1e-6 * ms
+            assert all(km[c] > 0 for c in range(9) if km[9 + c] > 0)
     finally:
         u1.set_pair_fusion(1)
+        u1.set_pass_fusion(1)
         u1.tune_enable(True)
-    # metropolis, or, or+plaq, metropolis pair, or pair, or pair+plaq (launches per sweep)
-    assert list(got[0][1][6:12]) == [8, 21, 3, 0, 0, 0]
-    assert list(got[2][1][6:12]) == [4, 10, 2, 2, 5, 1]
-    assert np.array_equal(got[0][2], got[2][2]) or np.max(np.abs(got[0][2] - got[2][2])) <= 1e-12
+    # launches per sweep: metropolis, or, or+plaq, metropolis pair, or pair,
+    # or pair+plaq, metropolis pass, or pass, or pass+plaq
+    assert list(got["plain"][1][9:18]) == [8, 21, 3, 0, 0, 0, 0, 0, 0]
+    assert list(got["pairs"][1][9:18]) == [4, 10, 2, 2, 5, 1, 0, 0, 0]
+    assert list(got["passes"][1][9:18]) == [0, 0, 0, 0, 0, 0, 2, 5, 1]
+    for mode in ("pairs", "passes"):
+        assert np.array_equal(got["plain"][2], got[mode][2]) or \
+            np.max(np.abs(got["plain"][2] - got[mode][2])) <= 1e-12
 
 
 @pytest.mark.parametrize("tune,graph", [(False, True), (True, True), (False, False)])
