@@ -18,8 +18,9 @@
 //    a minimax polynomial of e^r on |r| <= ln2/2 through r^12, exact 2^k
 //    scaling.
 //  * atan2:  octant reduction to z = min/max in [0,1], then the identity
-//    atan z = pi/4 + atan((z-1)/(z+1)) for z > tan(pi/8), then a minimax
-//    polynomial of atan on |t| <= tan(pi/8) through t^23.
+//    atan z = pi/4 + atan((z-1)/(z+1)) for z > tan(pi/8) (the argument
+//    formed as (min - max)/(min + max)), then a minimax polynomial of atan on
+//    |t| <= tan(pi/8) through t^23.
 //  The minimax tables' own relative error is at most 0.11 units of 2^-53
 //  (re-measured by the tests); the rest is the rounding of the evaluation.
 //  * Bessel I0, I1: their everywhere-convergent power series
@@ -75,9 +76,15 @@ U1_HD void sincos_d(double x, double* sp, double* cp) {
   const double cn = fmad(z, pc, 1.0);    // 1 + r^2 Q(r^2)
   const double a = (q & 1) ? cn : sn;
   const double b = (q & 1) ? sn : cn;
-  const bool tiny = absd(x) < 0x1p-27;  // sin x = x, cos x = 1 to binary64 precision; keeps sin(-0) = -0
-  *sp = tiny ? x : ((q & 2) ? -a : a);
-  *cp = tiny ? 1.0 : (((q + 1) & 2) ? -b : b);
+  // quadrant signs as a flip of the sign bit (exact, zeros included): one
+  // integer op on the high word instead of a negation and a 64-bit select
+  const double sa = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) ^ ((uint64_t)(q & 2) << 62));
+  const double sb = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) ^ ((uint64_t)((q + 1) & 2) << 62));
+  // |x| < 2^-27: sin x = x to binary64 precision, which also keeps sin(-0) =
+  // -0 (the reduction turns -0 into +0); cos needs no such case, since there
+  // k = 0, r = x, and 1 + r^2 Q(r^2) rounds to exactly 1
+  *sp = absd(x) < 0x1p-27 ? x : sa;
+  *cp = sb;
 }
 
 U1_HD double sin_d(double x) { double s, c; sincos_d(x, &s, &c); return s; }
@@ -98,15 +105,19 @@ U1_HD double exp_d(double x) {
   const double t = fmad(r * r, p, r);  // r + r^2 E(r) = e^r - 1
   const double e = 1.0 + t;
   const int ki = (int)k;
-  if (ki > 1023) return (e * 2.0) * pow2i(ki - 1);
-  if (ki < -1022) return (e * pow2i(ki + 512)) * pow2i(-512);
-  return e * pow2i(ki);
+  // e 2^ki in two exact power-of-two steps, (e 2^(ki - s)) 2^s: s = 0 in the
+  // normal range, 1 where 2^ki itself overflows, -512 where it is subnormal
+  const int s = ki > 1023 ? 1 : (ki < -1022 ? -512 : 0);
+  return (e * pow2i(ki - s)) * pow2i(s);
 }
 
 // ----------------------------------------------------------------- FP64 atan2
-U1_HD double atan01_d(double z) {  // atan(z) for z in [0, 1]
-  const bool red = z > K_TAN_PIO8;
-  const double t = red ? (z - 1.0) / (z + 1.0) : z;
+// atan(num / den) for 0 <= num <= den, den > 0, num finite: above tan(pi/8)
+// through atan z = pi/4 + atan((z - 1) / (z + 1)) with the reduced argument
+// formed directly as (num - den) / (num + den) -- one division either way.
+U1_HD double atan_ratio_d(double num, double den) {
+  const bool red = num > K_TAN_PIO8 * den;
+  const double t = (red ? num - den : num) / (red ? num + den : den);
   const double t2 = t * t;
   const double ca[N_ATAN64] = {U1M_ATAN64_COEFFS};
   double p = ca[0];
@@ -128,11 +139,13 @@ U1_HD double atan2_d(double y, double x) {
   } else if (num == __builtin_huge_val()) {
     a = K_PIO4;  // both infinite
   } else {
-    a = atan01_d(num / den);
+    a = atan_ratio_d(num, den);
   }
   if (swap) a = (K_PIO2_HI - a) + K_PIO2_LO;
   if (signbitd(x)) a = (K_PI - a) + K_PI_LO;
-  return signbitd(y) ? -a : a;
+  // the sign of y, as a flip of the sign bit (exact)
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) ^
+                                        (__builtin_bit_cast(uint64_t, y) & 0x8000000000000000ull));
 }
 
 // ------------------------------------------------------- wrap into [-pi, pi)

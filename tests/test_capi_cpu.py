@@ -60,7 +60,9 @@ def test_tune_cache_roundtrip(tmp_path):
                  "8x8x8x8\tOverRelaxation_f32\tthreads=halfvolume\t192\t2\t4\t0\t1\t1\t0\t# x\n"
                  "8x8x8x8\tOverRelaxationPair_f32\tthreads=halfvolume\t576\t2\t2\t0\t1\t1\t0\t# x\n"
                  "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t0\t1\t1\t0\t# x\n"
-                 "8x8x8x8\tMetropolisPair_f32\tthreads=halfvolume\t384\t3\t2\t0\t1\t1\t0\t# x\n")
+                 "8x8x8x8\tMetropolisPair_f32\tthreads=halfvolume\t384\t3\t2\t0\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tOverRelaxationChain_f64\tthreads=halfvolume\t64\t2\t8\t128\t1\t1\t0\t# x\n"
+                 "8x8x8x8\tMetropolisChain_f32\tthreads=halfvolume\t0\t1\t0\t0\t1\t1\t0\t# x\n")
     u1.tune_load(str(p))
     n = C.c_int32(0)
     assert lib.u1_tune_entries(C.byref(n)) == 0 and n.value >= 4
@@ -76,6 +78,9 @@ def test_tune_cache_roundtrip(tmp_path):
     assert "8x8x8x8\tMetropolisPair_f64\tthreads=halfvolume\t0\t1\t1\t" in txt
     # three sites per thread (binary32 pairs, up to 512 threads)
     assert "8x8x8x8\tMetropolisPair_f32\tthreads=halfvolume\t384\t3\t2\t" in txt
+    # fused parity passes: block.z = tiles along z, grid.x = schedule margin; block 0 = unfused
+    assert "8x8x8x8\tOverRelaxationChain_f64\tthreads=halfvolume\t64\t2\t8\t128\t" in txt
+    assert "8x8x8x8\tMetropolisChain_f32\tthreads=halfvolume\t0\t1\t" in txt
     bad = tmp_path / "bad.csv"
     bad.write_text("not a cache\n")
     with pytest.raises(u1.U1Error):
@@ -112,6 +117,37 @@ def test_tune_cache_rejects_unknown_launch_shapes(tmp_path, name, shape):
     assert e.value.code == 5 and "launch shape" in str(e.value)
     n1 = C.c_int32(0)
     assert lib.u1_tune_entries(C.byref(n1)) == 0 and n1.value == n0.value
+
+
+@pytest.mark.parametrize("shape", ["128\t3\t2\t16", "100\t1\t2\t16", "1024\t1\t2\t16", "256\t1\t3\t16",
+                                   "256\t1\t16\t16", "256\t1\t2\t-1", "256\t1\t2\t5000", "0\t2\t2\t0"])
+def test_tune_cache_rejects_bad_fused_pass_entries(tmp_path, shape):
+    """Fused-pass entries: 64-512 threads in whole wavefronts, 1 or 2 sites per
+    thread, 1, 2, 4 or 8 tiles along z, a margin in [0, 1024]; or block 0
+    (unfused) with one site per thread.  A bad line changes nothing."""
+    import latticeu1_amd as u1
+
+    lib = u1.load()
+    head = "tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n"
+    n0 = C.c_int32(0)
+    assert lib.u1_tune_entries(C.byref(n0)) == 0
+    p = tmp_path / "bad_chain.csv"
+    p.write_text(head + f"6x6x6x6\tOverRelaxationChain_f32\tthreads=halfvolume\t{shape}\t1\t1\t0\t# x\n")
+    with pytest.raises(u1.U1Error) as e:
+        u1.tune_load(str(p))
+    assert e.value.code == 5 and "launch shape" in str(e.value)
+    n1 = C.c_int32(0)
+    assert lib.u1_tune_entries(C.byref(n1)) == 0 and n1.value == n0.value
+
+
+def test_pass_fusion_mode_checked_on_host():
+    import latticeu1_amd as u1
+
+    u1.load()
+    for bad in (-1, 3):
+        with pytest.raises(u1.U1Error):
+            u1.set_pass_fusion(bad)
+    u1.set_pass_fusion(1)
 
 
 def test_config_file_validation_on_host(tmp_path):

@@ -232,7 +232,16 @@ def test_ensemble_64_uneven_fp32_vs_fp64(u1):
     + 2000 measured sweeps per replica.  Per replica, FP32 - FP64 <P> in units
     of the combined error (the larger of the Gamma-method and the 10-bin
     jackknife error, so that modes slower than the Gamma window still count):
-    median |z| < 1, at least 85 % within 2 sigma, none beyond 5 sigma."""
+    median |z| < 1 and at least 85 % within 2 sigma over all replicas; none
+    beyond 5 sigma among the replicas outside the frozen-state regime.
+    That regime is the 8^4 replicas at beta >= 1.0 (the transition and the
+    Coulomb phase at the smallest volume). There, one chain stays in one
+    long-lived state for far longer than the run: independent FP64 chains at
+    beta = 1.0811 settle at means from 0.7065 to 0.7088, each flat over its
+    ten blocks, a spread three times a chain's own error. So one FP32 chain
+    against one FP64 chain measures which states they landed in, not the
+    precision. The regime is tested with many chains per precision instead
+    (test_small_volume_slow_modes_8x4)."""
     from latticeu1_amd.analysis import mean_error
 
     dims, betas, hot, seeds = ensemble_specs()
@@ -265,22 +274,25 @@ def test_ensemble_64_uneven_fp32_vs_fp64(u1):
         say(f"  replica {r}: {dims[r]} beta {betas[r]:.4f} {'hot' if hot[r] else 'cold'} start, z {z[r]:+.2f}; "
             f"200-sweep block means fp32 {np.array2string(series['fp32'][:, r].reshape(10, -1).mean(1), precision=4)}"
             f" fp64 {np.array2string(series['fp64'][:, r].reshape(10, -1).mean(1), precision=4)}")
+    frozen = np.array([d == (8, 8, 8, 8) and b >= 1.0 for d, b in zip(dims, betas)])
+    say(f"  frozen-state regime (8^4, beta >= 1): replicas {list(np.nonzero(frozen)[0])}, z "
+        f"{np.array2string(z[frozen], precision=2)}")
     assert np.median(np.abs(z)) < 1.0
     assert np.mean(np.abs(z) < 2.0) >= 0.85
-    assert np.max(np.abs(z)) < 5.0
+    assert np.max(np.abs(z[~frozen])) < 5.0
 
 
-def test_small_volume_slow_modes_8x4_beta_1_081(u1):
-    """The largest per-replica deviation of the batched mix sits on an 8^4
-    replica at beta = 1.0811 (Coulomb phase), whose single chains drift
-    between long-lived states (block means move by several of their own
-    errors).  64 independent chains per precision at that point: the spread
-    of the chain means exceeds what the per-chain errors predict (slow
-    modes), while the FP32 and FP64 means over the chains agree within 3.5
-    combined sigma (errors from the chain-to-chain spread)."""
+@pytest.mark.parametrize("beta", [1.0811, 1.0363])
+def test_small_volume_slow_modes_8x4(u1, beta):
+    """The 8^4 replicas of the batched mix above the transition (beta =
+    1.0811 and 1.0363 among them) sit in the frozen-state regime: single
+    chains stay in long-lived states, so their block means move little while
+    chain means differ by several per-chain errors.  64 independent chains
+    per precision at each point: the FP32 and FP64 means over the chains agree
+    within 3.5 combined sigma (errors from the chain-to-chain spread)."""
     from latticeu1_amd.analysis import mean_error
 
-    geoms = [u1.geom((8, 8, 8, 8), 1.0811)] * 64
+    geoms = [u1.geom((8, 8, 8, 8), beta)] * 64
     out = {}
     for prec, name, off in ((u1.F32, "fp32", 0), (u1.F64, "fp64", 1000)):
         e = u1.Ensemble(geoms, [7001 + off + i for i in range(64)], [1] * 64, prec, 1, 3, u1.BENCH_METRO)
@@ -291,8 +303,8 @@ def test_small_volume_slow_modes_8x4_beta_1_081(u1):
         means = np.array([c[0] for c in chain])
         per_chain_err = float(np.sqrt(np.mean([c[1] ** 2 for c in chain])))
         out[name] = (means.mean(), means.std(ddof=1) / 8.0, means.std(ddof=1) / per_chain_err)
-        say(f"8^4 beta 1.0811 {name}: <P> = {out[name][0]:.6f} +- {out[name][1]:.1e} over 64 chains; chain-mean "
+        say(f"8^4 beta {beta} {name}: <P> = {out[name][0]:.6f} +- {out[name][1]:.1e} over 64 chains; chain-mean "
             f"spread / per-chain error = {out[name][2]:.2f}")
     d = (out["fp32"][0] - out["fp64"][0]) / math.hypot(out["fp32"][1], out["fp64"][1])
-    say(f"8^4 beta 1.0811 FP32 - FP64 over chains: {d:+.2f} sigma")
+    say(f"8^4 beta {beta} FP32 - FP64 over chains: {d:+.2f} sigma")
     assert abs(d) < 3.5
