@@ -253,6 +253,12 @@ template <>
 struct UMath<double> {
   static __device__ __forceinline__ void sincos(double x, double* s, double* c) { u1m::sincos_d(x, s, c); }
   static __device__ __forceinline__ double exp(double x) { return u1m::exp_d(x); }
+  // the same functions for calls inside a loop (coefficients pinned to
+  // scalar registers, u1m::kc; identical results)
+  static __device__ __forceinline__ void sincos_in_loop(double x, double* s, double* c) {
+    u1m::sincos_d<true>(x, s, c);
+  }
+  static __device__ __forceinline__ double exp_in_loop(double x) { return u1m::exp_d<true>(x); }
   static __device__ __forceinline__ double atan2(double y, double x) { return u1m::atan2_d(y, x); }
 };
 template <>

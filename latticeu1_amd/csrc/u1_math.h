@@ -42,6 +42,22 @@
 
 namespace u1m {
 
+// A polynomial coefficient as an operand of its FMA.  With PIN on the GPU an
+// empty asm statement ties each use to a scalar register pair, so that the
+// compiler neither hoists the constant out of a surrounding loop into a
+// vector register nor, in turn, copies it into the accumulator before a
+// two-address FMA (one more vector instruction per polynomial step); the
+// scalar moves that rebuild it run on the scalar unit.  For calls inside a
+// loop (the Metropolis hits); elsewhere the extra scalar registers would
+// only spill.  The value is unchanged, so are the results.
+template <bool PIN>
+U1_HD double kc(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (PIN) asm volatile("" : "+s"(c));
+#endif
+  return c;
+}
+
 U1_HD double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 U1_HD float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 U1_HD double rintd(double x) { return __builtin_rint(x); }
@@ -57,6 +73,8 @@ U1_HD double pow2i(int k) {
 }
 
 // ---------------------------------------------------------------- FP64 sincos
+// PIN: see kc (the same results either way).
+template <bool PIN = false>
 U1_HD void sincos_d(double x, double* sp, double* cp) {
   const double k = rintd(x * K_TWO_OVER_PI);
   double r = fmad(-k, K_PIO2_1, x);
@@ -68,10 +86,10 @@ U1_HD void sincos_d(double x, double* sp, double* cp) {
   const double cc_[N_COS64] = {U1M_COS64_COEFFS};
   double ps = cs_[0];
 #pragma unroll
-  for (int i = 1; i < N_SIN64; ++i) ps = fmad(ps, z, cs_[i]);
+  for (int i = 1; i < N_SIN64; ++i) ps = fmad(ps, z, kc<PIN>(cs_[i]));
   double pc = cc_[0];
 #pragma unroll
-  for (int i = 1; i < N_COS64; ++i) pc = fmad(pc, z, cc_[i]);
+  for (int i = 1; i < N_COS64; ++i) pc = fmad(pc, z, kc<PIN>(cc_[i]));
   const double sn = fmad(r * z, ps, r);  // r + r^3 P(r^2)
   const double cn = fmad(z, pc, 1.0);    // 1 + r^2 Q(r^2)
   const double a = (q & 1) ? cn : sn;
@@ -91,6 +109,7 @@ U1_HD double sin_d(double x) { double s, c; sincos_d(x, &s, &c); return s; }
 U1_HD double cos_d(double x) { double s, c; sincos_d(x, &s, &c); return c; }
 
 // ------------------------------------------------------------------- FP64 exp
+template <bool PIN = false>
 U1_HD double exp_d(double x) {
   if (x != x) return x;
   if (x > 709.782712893384) return __builtin_huge_val();
@@ -101,7 +120,7 @@ U1_HD double exp_d(double x) {
   const double ce[N_EXP64] = {U1M_EXP64_COEFFS};
   double p = ce[0];
 #pragma unroll
-  for (int i = 1; i < N_EXP64; ++i) p = fmad(p, r, ce[i]);
+  for (int i = 1; i < N_EXP64; ++i) p = fmad(p, r, kc<PIN>(ce[i]));
   const double t = fmad(r * r, p, r);  // r + r^2 E(r) = e^r - 1
   const double e = 1.0 + t;
   const int ki = (int)k;

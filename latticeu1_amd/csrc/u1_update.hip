@@ -159,9 +159,9 @@ __device__ __forceinline__ bool metro_hit(MetroState<Real>& m, Staple<Real> S, R
   }
   const Real tn = (Real)tnd;
   Real sn, cn;
-  UMath<Real>::sincos(tn, &sn, &cn);
+  UMath<Real>::sincos_in_loop(tn, &sn, &cn);
   const Real re_new = S.re * cn - S.im * sn;
-  const Real dS = UMath<Real>::exp(beta * (re_new - m.re_old));
+  const Real dS = UMath<Real>::exp_in_loop(beta * (re_new - m.re_old));
   const Real b = Prec<Real>::u01(wb);
   const bool ok = dS > b;
   if (ok) {
