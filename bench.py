@@ -133,9 +133,10 @@ def class_table(kms, V, s):
     return out
 
 
-def roofline_of(cls, sweep_ms):
+def roofline_of(cls, sweep_ms, real="float"):
     """Roofline object for the launch class with the largest time per sweep,
-    on its own algorithmic bytes, plus the sweep-level fraction."""
+    on its own algorithmic bytes, plus the sweep-level fraction (real: the
+    kernels' element type in the kernel name)."""
     dom = max(cls, key=lambda k: cls[k]["ms_per_sweep"])
     d = cls[dom]
     achieved = d["bytes_per_launch"] / (d["mean_launch_ms"] * 1e-3) / 1e9
@@ -143,7 +144,7 @@ def roofline_of(cls, sweep_ms):
     sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     return dom, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                 "kernel": KERNEL_NAMES[CLASSES.index(dom)], "class": dom,
+                 "kernel": KERNEL_NAMES[CLASSES.index(dom)].replace("<float", "<" + real), "class": dom,
                  "bytes_per_launch": d["bytes_per_launch"], "mean_launch_ms": d["mean_launch_ms"],
                  "time_share": d["time_share"], "launch_classes": cls,
                  "sweep_algorithmic_bytes": sweep_bytes, "sweep_algorithmic_GBs": round(sweep_gbs, 1),
@@ -203,7 +204,7 @@ def run_batched(u1, steps, warmup, skip_fp64):
         ms, kms, series = e.bench(steps, kernel_timing=True)
         s = 4 if prec == u1.F32 else 8
         cls = class_table(kms, vol, s)
-        _, roof = roofline_of(cls, ms / steps)
+        _, roof = roofline_of(cls, ms / steps, "float" if prec == u1.F32 else "double")
         pm = np.array([[u1.mean_plaquette(r) for r in sw] for sw in series])
         out[name] = {"link_updates_per_s": 16.0 * vol * steps / (ms * 1e-3), "ms_per_sweep": ms / steps,
                      "roofline": roof, "mean_plaquette_over_replicas": float(pm[-1].mean())}

@@ -174,8 +174,9 @@ __device__ __forceinline__ bool metro_hit(MetroState<Real>& m, Staple<Real> S, R
 // The same hit in binary32, from the same two words.  The proposal is
 // a (2 wu 2^-32 - 1) as one FMA, with a = pi (independent: pscale = pi 2^-31,
 // poff = pi) or a = step (theta + delta).  theta + delta is not wrapped here:
-// the caller wraps theta after each pair of hits (|theta| <= 3 pi in between,
-// well inside the hardware sin/cos range).  The accept test
+// the caller wraps theta (x - 2 pi rint(x / 2 pi), any range) after every
+// second pair of hits and after the last (|theta| <= pi + 4 step <= 5 pi in
+// between, well inside the hardware sin/cos range).  The accept test
 // exp(beta dRe) > wb 2^-32 is evaluated as 2^(beta log2(e) dRe + 32) > wb
 // (bl2e = beta log2(e)): one v_exp_f32 and the word converted without scaling.
 // thm is 1 for theta + delta and 0 for the independent proposal, so both
@@ -279,6 +280,60 @@ __device__ __forceinline__ void phase_load_nu0_lds(const Real* lds_th, uint32_t 
   }
 }
 
+// Phase (parity, 1) of a fused pair reads seven angles that phase (parity, 0)
+// of the same site has already loaded, at the same addresses, and that
+// neither phase writes: the link's own theta_1(x) (c of nu = 1 in phase 0),
+// a = theta_0(x + e1) and b = theta_1(x + e0) of nu = 0 (b and a of nu = 1 in
+// phase 0), and c = theta_nu(x), f = theta_nu(x - e_nu) of nu = 2, 3 (the
+// same slots j = 1, 2 in both phases).  pair_shared keeps them from phase 0's
+// staple angles, pair_take hands them to phase 1.
+template <typename Real>
+struct PairShared {
+  Real th, a0, b0, c2, f2, c3, f3;
+};
+template <typename Real>
+__device__ __forceinline__ PairShared<Real> pair_shared(const StapleIn<Real>& in0) {
+  return {in0.c[0], in0.b[0], in0.a[0], in0.c[1], in0.f[1], in0.c[2], in0.f[2]};
+}
+template <typename Real>
+__device__ __forceinline__ void pair_take(const PairShared<Real>& s, StapleIn<Real>& in1, Real& th1) {
+  th1 = s.th;
+  in1.a[0] = s.a0;
+  in1.b[0] = s.b0;
+  in1.c[1] = s.c2;
+  in1.f[1] = s.f2;
+  in1.c[2] = s.c3;
+  in1.f[2] = s.f3;
+}
+
+// The phase-(parity, 1) angles of the sites in q.ps that neither the LDS
+// exchange (c, d of nu = 0) nor pair_take provides: e, f of nu = 0 and a, b,
+// d, e of nu = 2, 3 -- ten loads instead of seventeen (the addresses of
+// staple_load for MU = 1).
+template <typename Real, int SPT>
+__device__ __forceinline__ void pair1_load(const Real* __restrict__ lat_base, const PhaseArgs& a, const RepDesc& r,
+                                           PhaseIn<Real, SPT>& q) {
+  const Real* lat = lat_base + r.lat_off;
+  const Real* p0 = lat;
+  const Real* p1 = lat + (size_t)r.V;
+#pragma unroll
+  for (int k = 0; k < SPT; ++k) {
+    const PhaseSite& s = q.ps[k];
+    q.same[k] = (uint32_t)(a.parity * r.HV) + s.id;
+    const uint32_t opp = (uint32_t)((1 - a.parity) * r.HV) + s.id;
+    q.in[k].e[0] = ld(p1, opp + s.bwd[0]);
+    q.in[k].f[0] = ld(p0, opp + s.bwd[0]);
+#pragma unroll
+    for (int nu = 2; nu < 4; ++nu) {
+      const Real* pn = lat + (size_t)nu * r.V;
+      q.in[k].a[nu - 1] = ld(pn, opp + s.fwd[1]);
+      q.in[k].b[nu - 1] = ld(p1, opp + s.fwd[nu]);
+      q.in[k].d[nu - 1] = ld(pn, q.same[k] + s.fwd[1] + s.bwd[nu]);
+      q.in[k].e[nu - 1] = ld(p1, opp + s.bwd[nu]);
+    }
+  }
+}
+
 // lds_th (nullable): the new angles are also written to lds_th[id - pbase],
 // the site's index within its (z, t) plane (fused pairs, phase 0).  COH: the
 // new angles are stored at device scope (fused passes).
@@ -335,7 +390,9 @@ __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const
             const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[2], w.v[3]);
             acc_wave += count_wave(ok1, vmask[k]);
           }
-          m[k].th = (float)u1m::wrap_near_d((double)m[k].th);  // back into [-pi, pi] once per pair
+          // back into [-pi, pi] after every second pair of hits and after the
+          // last (|theta| <= pi + 4 step in between)
+          if ((h & 2) || h + 2 >= a.nhit) m[k].th = (float)u1m::wrap_near_d((double)m[k].th);
         } else {
           const bool ok0 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
           acc_wave += count_wave(ok0, vmask[k]);
@@ -438,9 +495,13 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 //  * nothing else either phase reads is written by the launch.
 // The pair reads the lattice once (4V angles) and writes V/2 + V/2: 5V angles
 // against 9V for two launches.  MEAS applies to phase (parity, 1).  Dynamic
-// LDS: P angles (pair_lds_bytes).  When one step covers the plane (P <= SPT *
-// blockDim.x, SPT <= 3), every load of phase 1 is issued before phase 0
-// computes, so phase 1 waits on the barrier and two LDS reads only.
+// LDS: P angles (pair_lds_bytes).  Phase 1 loads ten of its nineteen angles:
+// two come from LDS and seven from phase 0's loads of the same site
+// (pair_shared).  When one step covers the plane (P <= SPT * blockDim.x, SPT
+// <= 3), every load of phase 1 is issued before phase 0 computes, so phase 1
+// waits on the barrier and two LDS reads only; with two steps (SPT = 1, not
+// measuring) the seven shared angles of both steps are kept across the
+// barrier.
 template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
 __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real* __restrict__ lat_base, PhaseArgs a) {
   extern __shared__ __align__(8) unsigned char pair_lds[];
@@ -463,11 +524,39 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
 #pragma unroll
     for (int k = 0; k < SPT; ++k) q1.ps[k] = q0.ps[k];  // the same sites
     phase_load<Real, 0, SPT>(lat_base, a, r, q0);
-    phase_load<Real, 1, SPT, 1>(lat_base, a, r, q1);
+    pair1_load<Real, SPT>(lat_base, a, r, q1);
     phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q0, v0, acc0, th0, pbase);
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) pair_take(pair_shared(q0.in[k]), q1.in[k], q1.th[k]);
     __syncthreads();
     phase_load_nu0_lds<Real, SPT>(th0, pbase, q1);
     phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q1, v, acc1);
+  } else if (SPT == 1 && !MEAS && P <= 2 * step) {
+    // two steps per phase: the angles phase 1 shares with phase 0 stay in
+    // registers across the barrier (pair_shared), one set per step (the
+    // measuring pair keeps the loop below: measured faster there)
+    PhaseIn<Real, SPT> q;
+    PairShared<Real> sh[2][SPT];
+    uint32_t pbase = 0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (phase_sites<0, SPT>(a, r, b, s * step + threadIdx.x, step, q.ps)) return;  // frozen plane
+      pbase = ((uint32_t)q.ps[0].z + (uint32_t)r.N[2] * q.ps[0].t) * P;
+      phase_load<Real, 0, SPT>(lat_base, a, r, q);
+      phase_compute<Real, 0, KIND, SPT, false>(lat_base, a, r, q, v0, acc0, th0, pbase);
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) sh[s][k] = pair_shared(q.in[k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      phase_sites<1, SPT>(a, r, b, s * step + threadIdx.x, step, q.ps);
+      pair1_load<Real, SPT>(lat_base, a, r, q);
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) pair_take(sh[s][k], q.in[k], q.th[k]);
+      phase_load_nu0_lds<Real, SPT>(th0, pbase, q);
+      phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
+    }
   } else {
     uint32_t pbase = 0;
     for (uint32_t base = 0; base < P; base += step) {

@@ -493,8 +493,9 @@ def test_fp32_overrelaxation_tracks_fp64(u1, orc, aniso):
                                                            (1, 4, 0.5, 1.0, 2.0)])
 def test_fp32_metropolis_tracks_fp64(u1, orc, proposal, nhit, step, beta, aniso):
     """The binary32 hit (proposal FMA, 2^(beta log2(e) dRe + 32) > raw word,
-    binary64 wrap per pair of hits) reads the same Philox words as the
-    binary64 hit, which matches the oracle bit for bit.  From a configuration
+    binary64 wrap after every second pair of hits and after the last) reads
+    the same Philox words as the binary64 hit, which matches the oracle bit
+    for bit.  From a configuration
     exactly representable in FP32, one Metropolis pass in each precision: the
     accept decisions agree except for the rare draws within rounding of the
     threshold (and the neighbours they perturb later in the pass), and the
