@@ -361,20 +361,27 @@ int phase_cfg(int prec, int kind, const u1_geom* g, const void* lat, const Phase
 // Fused pair shapes of a geometry: whole (z, t) planes per workgroup, SPT
 // sites per thread per step, the workgroup size for one step over the plane
 // and for two, rounded up to whole wavefronts and capped at the kernel's
-// bound (larger planes take more steps).
+// bound (larger planes take more steps).  With one site per thread, also
+// 1.25 and 1.5 times the two-step size: a full first step and a shorter
+// second one.
 std::vector<Cfg> pair_candidates(int prec, const RepDesc& r) {
   const int P = r.nxh * r.N[1];
   std::vector<Cfg> v;
+  auto add = [&v](int b, int spt) {
+    for (const Cfg& c : v)
+      if (c.block == b && c.spt == spt) return;
+    for (int t = 0; t < 2; ++t)
+      for (int x = 0; x < 2; ++x) v.push_back(Cfg{b, spt, x, t});
+  };
   for (int spt : {1, 2, 3, 4}) {
     if (spt >= 3 && prec != U1_F32) continue;
     const int maxb = u1d::pair_max_block((int)elem_size(prec), spt);
-    int prev = 0;
-    for (int steps = 1; steps <= 2; ++steps) {
-      const int b = std::min(maxb, ((P + spt * steps - 1) / (spt * steps) + 63) / 64 * 64);
-      if (b == prev) continue;
-      prev = b;
-      for (int t = 0; t < 2; ++t)
-        for (int x = 0; x < 2; ++x) v.push_back(Cfg{b, spt, x, t});
+    for (int steps = 1; steps <= 2; ++steps)
+      add(std::min(maxb, ((P + spt * steps - 1) / (spt * steps) + 63) / 64 * 64), spt);
+    if (spt == 1) {
+      const int b2 = (P + 1) / 2;
+      for (int q : {5, 6})  // b2 * q / 4
+        if (b2 * q / 4 < maxb) add((b2 * q / 4 + 63) / 64 * 64, spt);
     }
   }
   return v;

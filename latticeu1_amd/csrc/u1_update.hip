@@ -495,13 +495,17 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 //  * nothing else either phase reads is written by the launch.
 // The pair reads the lattice once (4V angles) and writes V/2 + V/2: 5V angles
 // against 9V for two launches.  MEAS applies to phase (parity, 1).  Dynamic
-// LDS: P angles (pair_lds_bytes).  Phase 1 loads ten of its nineteen angles:
-// two come from LDS and seven from phase 0's loads of the same site
-// (pair_shared).  When one step covers the plane (P <= SPT * blockDim.x, SPT
-// <= 3), every load of phase 1 is issued before phase 0 computes, so phase 1
-// waits on the barrier and two LDS reads only; with two steps (SPT = 1, not
-// measuring) the seven shared angles of both steps are kept across the
-// barrier.
+// LDS: P angles (pair_lds_bytes).  Of phase 1's nineteen angles two come
+// from LDS; in the two forms below that keep a thread's sites over both
+// phases, seven more come from phase 0's loads of the same site
+// (pair_shared) and phase 1 loads ten:
+//  * one step covers the plane (P <= SPT * blockDim.x, SPT <= 3): every load
+//    of phase 1 is issued before phase 0 computes, so phase 1 waits on the
+//    barrier and two LDS reads only;
+//  * two steps (SPT = 1, not measuring): the seven shared angles of both
+//    steps are kept in registers across the barrier.
+// Otherwise (more steps, or the measuring pair in two) phase 1 loads its
+// seventeen angles after the barrier.
 template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
 __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real* __restrict__ lat_base, PhaseArgs a) {
   extern __shared__ __align__(8) unsigned char pair_lds[];

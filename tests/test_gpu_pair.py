@@ -106,13 +106,15 @@ def test_fp32_pair_matches_two_launches(u1, dims, tune):
 
 
 @pytest.mark.parametrize("block,spt,bz", [(384, 3, 2), (576, 2, 2), (384, 3, 1), (576, 2, 3), (320, 4, 2),
-                                          (576, 1, 2), (1024, 1, 4), (192, 4, 2), (128, 3, 2)])
+                                          (576, 1, 2), (768, 1, 4), (896, 1, 2), (1024, 1, 4), (192, 4, 2),
+                                          (128, 3, 2)])
 def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt, bz):
     """Every fused-pair shape the tuner can choose on a 48 x 48 plane
     (P = 1152 sites per parity: one exact step of 384 x 3 or 576 x 2; one
-    step with idle slots; two or more steps), in lexicographic, XCD-ordered
-    and tiled plane orders (block.z 1 to 4), forced through the tune cache,
-    bit for bit against two launches in binary32."""
+    step with idle slots; two steps, the second full or shorter; more
+    steps), in lexicographic, XCD-ordered and tiled plane orders (block.z 1
+    to 4), forced through the tune cache, bit for bit against two launches in
+    binary32."""
     dims = (48, 48, 4, 6)
     vol = "x".join(map(str, dims))
     cache = tmp_path / "shapes.csv"
