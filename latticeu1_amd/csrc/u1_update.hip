@@ -502,11 +502,19 @@ __global__ __launch_bounds__(512) void k_phase(Real* __restrict__ lat_base, Phas
 //  * one step covers the plane (P <= SPT * blockDim.x, SPT <= 3): every load
 //    of phase 1 is issued before phase 0 computes, so phase 1 waits on the
 //    barrier and two LDS reads only;
-//  * two steps (SPT = 1, not measuring): the seven shared angles of both
-//    steps are kept in registers across the barrier.
-// Otherwise (more steps, or the measuring pair in two) phase 1 loads its
-// seventeen angles after the barrier.
-template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false>
+//  * two steps (SPT = 1): the seven shared angles of both steps are kept in
+//    registers across the barrier.
+// Otherwise (more steps) phase 1 loads its seventeen angles after the
+// barrier.
+// FORM picks the form at compile time for a single lattice, whose plane
+// size the host knows (PAIR_ONE_STEP, PAIR_TWO_STEPS, PAIR_LOOP): each
+// kernel then holds the code of its own form only (in one kernel holding
+// all three, the compiler's schedule of the measuring two-step form took
+// 42 us where its own kernel takes 30 us at 48^4); PAIR_ANY (batched
+// replicas, planes of several sizes) decides per workgroup, without the
+// measuring two-step form.
+enum { PAIR_ONE_STEP = 0, PAIR_TWO_STEPS = 1, PAIR_LOOP = 2, PAIR_ANY = 3 };
+template <typename Real, int KIND, int SPT, bool MEAS, bool BATCHED = false, int FORM = PAIR_ANY>
 __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real* __restrict__ lat_base, PhaseArgs a) {
   extern __shared__ __align__(8) unsigned char pair_lds[];
   Real* th0 = reinterpret_cast<Real*>(pair_lds);
@@ -517,7 +525,10 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
   const uint32_t step = SPT * blockDim.x;
   double v0[2] = {0.0, 0.0}, v[2] = {0.0, 0.0};
   uint32_t acc0 = 0, acc1 = 0;
-  if (SPT <= 3 && P <= step) {
+  constexpr bool kOne = SPT <= 3 && (FORM == PAIR_ONE_STEP || FORM == PAIR_ANY);
+  constexpr bool kTwo = SPT == 1 && (FORM == PAIR_TWO_STEPS || (FORM == PAIR_ANY && !MEAS));
+  constexpr bool kLoop = FORM == PAIR_LOOP || FORM == PAIR_ANY;
+  if (kOne && (FORM != PAIR_ANY || P <= step)) {
     PhaseIn<Real, SPT> q0, q1;
     // a frozen plane (multilevel mask, uniform over the workgroup) is frozen
     // for both phases: the whole workgroup leaves before the barrier
@@ -535,10 +546,9 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
     __syncthreads();
     phase_load_nu0_lds<Real, SPT>(th0, pbase, q1);
     phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q1, v, acc1);
-  } else if (SPT == 1 && !MEAS && P <= 2 * step) {
+  } else if (kTwo && (FORM != PAIR_ANY || P <= 2 * step)) {
     // two steps per phase: the angles phase 1 shares with phase 0 stay in
-    // registers across the barrier (pair_shared), one set per step (the
-    // measuring pair keeps the loop below: measured faster there)
+    // registers across the barrier (pair_shared), one set per step
     PhaseIn<Real, SPT> q;
     PairShared<Real> sh[2][SPT];
     uint32_t pbase = 0;
@@ -561,7 +571,7 @@ __global__ __launch_bounds__(pair_max_block(sizeof(Real), SPT)) void k_pair(Real
       phase_load_nu0_lds<Real, SPT>(th0, pbase, q);
       phase_compute<Real, 1, KIND, SPT, MEAS>(lat_base, a, r, q, v, acc1);
     }
-  } else {
+  } else if (kLoop) {
     uint32_t pbase = 0;
     for (uint32_t base = 0; base < P; base += step) {
       PhaseIn<Real, SPT> q;
@@ -740,14 +750,37 @@ PhaseArgs with_f32_constants(const PhaseArgs& a) {
   return b;
 }
 
+template <typename Real, int KIND, int SPT, bool MEAS>
+void launch_pair_form(int form, Real* lat, const PhaseArgs& a, int nblocks, int block, size_t lds, hipStream_t s) {
+  if constexpr (SPT <= 3)
+    if (form == PAIR_ONE_STEP) {
+      k_pair<Real, KIND, SPT, MEAS, false, PAIR_ONE_STEP><<<nblocks, block, lds, s>>>(lat, a);
+      return;
+    }
+  if constexpr (SPT == 1)
+    if (form == PAIR_TWO_STEPS) {
+      k_pair<Real, KIND, SPT, MEAS, false, PAIR_TWO_STEPS><<<nblocks, block, lds, s>>>(lat, a);
+      return;
+    }
+  k_pair<Real, KIND, SPT, MEAS, false, PAIR_LOOP><<<nblocks, block, lds, s>>>(lat, a);
+}
+
 template <typename Real, int KIND, int SPT>
 hipError_t launch_pair_meas(Real* lat, const PhaseArgs& a, int nblocks, int block, size_t lds, hipStream_t s) {
-  if (a.reps != nullptr)  // batched replicas: no measuring pairs
+  if (a.reps != nullptr) {  // batched replicas: no measuring pairs; the form per workgroup
     k_pair<Real, KIND, SPT, false, true><<<nblocks, block, lds, s>>>(lat, a);
-  else if (a.meas != nullptr)
-    k_pair<Real, KIND, SPT, true><<<nblocks, block, lds, s>>>(lat, a);
+    return hipGetLastError();
+  }
+  // a single lattice: its plane size fixes the form (the kernel's own test)
+  const int64_t P = (int64_t)a.single.nxh * a.single.N[1], step = (int64_t)SPT * block;
+  const bool meas = a.meas != nullptr;
+  const int form = SPT <= 3 && P <= step ? PAIR_ONE_STEP
+                   : SPT == 1 && P <= 2 * step ? PAIR_TWO_STEPS
+                                                         : PAIR_LOOP;
+  if (meas)
+    launch_pair_form<Real, KIND, SPT, true>(form, lat, a, nblocks, block, lds, s);
   else
-    k_pair<Real, KIND, SPT, false><<<nblocks, block, lds, s>>>(lat, a);
+    launch_pair_form<Real, KIND, SPT, false>(form, lat, a, nblocks, block, lds, s);
   return hipGetLastError();
 }
 
