@@ -71,14 +71,14 @@ KERNEL_NAMES = ("k_phase<float> Metropolis (parity, mu) phase",
                 "k_chain<float> fused overrelaxation parity pass (phases 0-3)",
                 "k_chain<float, MEAS> fused overrelaxation parity pass summing the plaquette")
 # kernel-trace names of each class (template arguments: phase <Real, MU, KIND,
-# BATCHED, SPT, MEAS>, pair <Real, KIND, SPT, MEAS, BATCHED>, pass <Real, KIND,
-# SPT, MEAS>; KIND 0 Metropolis, 1 overrelaxation)
+# BATCHED, SPT, MEAS>, pair <Real, KIND, SPT, MEAS, BATCHED, FORM>, pass <Real,
+# KIND, SPT, MEAS>; KIND 0 Metropolis, 1 overrelaxation)
 TRACE_PATTERNS = (r"k_phase<float, \d, 0, (false|true), \d, false>",
                   r"k_phase<float, \d, 1, (false|true), \d, false>",
                   r"k_phase<float, \d, 1, (false|true), \d, true>",
-                  r"k_pair<float, 0, \d, false(, (false|true))?>",
-                  r"k_pair<float, 1, \d, false(, (false|true))?>",
-                  r"k_pair<float, 1, \d, true(, (false|true))?>",
+                  r"k_pair<float, 0, \d, false(, (false|true)(, \d)?)?>",
+                  r"k_pair<float, 1, \d, false(, (false|true)(, \d)?)?>",
+                  r"k_pair<float, 1, \d, true(, (false|true)(, \d)?)?>",
                   r"k_chain<float, 0, \d, false>",
                   r"k_chain<float, 1, \d, false>",
                   r"k_chain<float, 1, \d, true>")
