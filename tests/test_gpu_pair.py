@@ -145,6 +145,42 @@ def test_fp32_pair_shapes_match_two_launches(u1, tmp_path, block, spt, bz):
     assert np.max(np.abs(out[0][2] - out[2][2])) <= 1e-12
 
 
+@pytest.mark.parametrize("block,form", [(512, "two steps"), (192, "more steps"), (64, "more steps, one wave")])
+def test_fp64_pair_forms_with_multilevel_mask(u1, orc, tmp_path, block, form):
+    """Multilevel masks (frozen planes leave before the barrier) through the
+    pair forms with steps: a 24 x 48 plane of 576 sites per parity in two
+    steps of 512 threads (phase 1 keeping the angles it shares with phase 0)
+    or in more steps, binary64 bit for bit against the oracle, eager sweeps
+    with two mask widths and a measuring sweep without a mask."""
+    dims = (24, 48, 2, 8)
+    vol = "x".join(map(str, dims))
+    cache = tmp_path / "forms.csv"
+    rows = [f"{vol}\t{name}_f64\tthreads=halfvolume\t{b}\t1\t2\t0\t1\t1\t0\t# forced"
+            for name, b in (("Metropolis", 64), ("OverRelaxation", 64), ("MetropolisPair", block),
+                            ("OverRelaxationPair", block))]
+    cache.write_text("tunecache\tlatticeu1_amd-1\tgpu_arch=gfx950\nvolume\tname\taux\n" + "\n".join(rows) + "\n")
+    try:
+        u1.tune_enable(False)
+        u1.set_pass_fusion(0)
+        u1.set_pair_fusion(1)
+        u1.tune_load(str(cache))
+        g, og, lat, ref = make_pair(u1, orc, dims, 1.0, 1.0, True, seed=31)
+        say(f"fp64 pair {form} ({block} threads) with multilevel masks {dims}")
+        om = orc.metro(1, 4, 0.5)
+        for ml, stream in ((4, 3), (2, 4)):
+            for s in range(2):
+                lat.update(1, 3, u1.BENCH_METRO, 31, s, stream=stream, c3=6, mask_ml=ml)
+                orc.update(og, ref, 1, 3, om, 31, s, stream=stream, c3=6, mask_ml=ml)
+            assert np.array_equal(bits(lat.to_host()), bits(ref)), ml
+        p = lat.update_plaquette(1, 3, u1.BENCH_METRO, 31, 2)
+        orc.update(og, ref, 1, 3, om, 31, 2)
+        assert np.array_equal(bits(lat.to_host()), bits(ref))
+        assert np.max(np.abs(p - orc.plaquette(og, ref))) <= 1e-13
+    finally:
+        u1.set_pass_fusion(1)
+        u1.tune_enable(True)
+
+
 def test_pair_fusion_mode_checked(u1):
     with pytest.raises(u1.U1Error):
         u1.set_pair_fusion(3)
