@@ -284,7 +284,9 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
  *    4: atan2_f((float)x, (float)y);  5: Bessel I0, I1 (out 2 per input);
  *    6: wrap_d(x);  7: hardware sincos of the FP32 update kernels (2 outputs);
  *    8: fast atan2 of the FP32 update kernels;  9: the wrap of the FP32
- *    update kernels, (float)wrap_near_d((float)x).  x, y, out are host arrays. */
+ *    update kernels, (float)wrap_near_d((float)x);  10: the exp of the
+ *    binary64 Metropolis hits (exp_d's bits by a per-wavefront fast path).
+ *    x, y, out are host arrays. */
 int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n);
 /* Philox4x32-10 on the device: ctr[4n], key[2n] -> out[4n] (host arrays). */
 int u1_selftest_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int64_t n);

@@ -2343,7 +2343,7 @@ int u1_bench_updates(const u1_geom* g, int prec, void* lat, int32_t metrop, int3
 // ---------------------------------------------------------------- self test
 int u1_selftest_math(int32_t fn, const double* x, const double* y, double* out, int64_t n) {
   REQ_INIT();
-  if (fn < 0 || fn > 9 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
+  if (fn < 0 || fn > 10 || !x || !out || n < 0) return fail(U1_ERR_ARG, "bad self-test arguments");
   if (n == 0) return U1_OK;
   const int nout = (fn == 0 || fn == 3 || fn == 5 || fn == 7) ? 2 : 1;
   DevBuf dx, dy, dout;

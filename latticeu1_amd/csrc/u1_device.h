@@ -258,7 +258,19 @@ struct UMath<double> {
   static __device__ __forceinline__ void sincos_in_loop(double x, double* s, double* c) {
     u1m::sincos_d<true>(x, s, c);
   }
-  static __device__ __forceinline__ double exp_in_loop(double x) { return u1m::exp_d<true>(x); }
+  // exp_d's bits for every x: exp_d_inrange, then exp_d's three special
+  // cases only in a wavefront where some lane's argument is outside
+  // |x| < 708 (one vote per hit instead of three tests; the Metropolis
+  // weights stay far inside at any usual coupling)
+  static __device__ __forceinline__ double exp_in_loop(double x) {
+    double y = u1m::exp_d_inrange<true>(x);
+    if (!__all(__builtin_fabs(x) < 708.0)) {
+      if (x != x) y = x;
+      else if (x > 709.782712893384) y = __builtin_huge_val();
+      else if (x < -745.1332191019412) y = 0.0;
+    }
+    return y;
+  }
   static __device__ __forceinline__ double atan2(double y, double x) { return u1m::atan2_d(y, x); }
 };
 template <>

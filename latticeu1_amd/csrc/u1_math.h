@@ -67,11 +67,6 @@ U1_HD float absf_(float x) { return __builtin_fabsf(x); }
 U1_HD bool signbitd(double x) { return __builtin_signbit(x); }
 U1_HD bool signbitf_(float x) { return __builtin_signbit(x); }
 
-// 2^k as an exact double for -1022 <= k <= 1023.
-U1_HD double pow2i(int k) {
-  return __builtin_bit_cast(double, (uint64_t)(int64_t)(k + 1023) << 52);
-}
-
 // ---------------------------------------------------------------- FP64 sincos
 // PIN: see kc (the same results either way).
 template <bool PIN = false>
@@ -109,11 +104,11 @@ U1_HD double sin_d(double x) { double s, c; sincos_d(x, &s, &c); return s; }
 U1_HD double cos_d(double x) { double s, c; sincos_d(x, &s, &c); return c; }
 
 // ------------------------------------------------------------------- FP64 exp
+// exp_d for -745.1332191019412 <= x <= 709.782712893384 (no tests): e^r
+// scaled by 2^k with ldexp, which rounds once where the result is subnormal
+// and overflows to infinity exactly where the product would.
 template <bool PIN = false>
-U1_HD double exp_d(double x) {
-  if (x != x) return x;
-  if (x > 709.782712893384) return __builtin_huge_val();
-  if (x < -745.1332191019412) return 0.0;
+U1_HD double exp_d_inrange(double x) {
   const double k = rintd(x * K_LOG2E);
   double r = fmad(-k, K_LN2_HI, x);
   r = fmad(-k, K_LN2_LO, r);
@@ -122,12 +117,15 @@ U1_HD double exp_d(double x) {
 #pragma unroll
   for (int i = 1; i < N_EXP64; ++i) p = fmad(p, r, kc<PIN>(ce[i]));
   const double t = fmad(r * r, p, r);  // r + r^2 E(r) = e^r - 1
-  const double e = 1.0 + t;
-  const int ki = (int)k;
-  // e 2^ki in two exact power-of-two steps, (e 2^(ki - s)) 2^s: s = 0 in the
-  // normal range, 1 where 2^ki itself overflows, -512 where it is subnormal
-  const int s = ki > 1023 ? 1 : (ki < -1022 ? -512 : 0);
-  return (e * pow2i(ki - s)) * pow2i(s);
+  return __builtin_ldexp(1.0 + t, (int)k);
+}
+
+template <bool PIN = false>
+U1_HD double exp_d(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  return exp_d_inrange<PIN>(x);
 }
 
 // ----------------------------------------------------------------- FP64 atan2

@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256) void k_math(int fn, const double* __restrict__
       break;
     }
     case 8: out[i] = UMath<float>::atan2((float)x[i], (float)y[i]); break;
+    case 10: out[i] = UMath<double>::exp_in_loop(x[i]); break;
     default: out[i] = (float)u1m::wrap_near_d((double)(float)x[i]); break;
   }
 }

@@ -50,6 +50,15 @@ def test_device_math_bit_identical(u1, orc):
     assert np.array_equal(bits(d[:, 0]), bits(s)) and np.array_equal(bits(d[:, 1]), bits(c))
     xe = x / 2
     assert np.array_equal(bits(u1.selftest_math(1, xe)), bits(orc.exp(xe)))
+    # exp at the ends of its range (subnormal results, overflow, NaN, inf),
+    # and the binary64 Metropolis hits' exp (fn 10), whose wavefronts with
+    # every |x| < 708 skip the range tests: in shuffled (mixed) wavefronts
+    # and in wavefronts wholly inside
+    xr = rng.permutation(np.concatenate([xe[:6000], rng.uniform(-750, -700, 3000), rng.uniform(700, 712, 3000),
+                                         [np.nan, np.inf, -np.inf, 709.782712893384, -745.1332191019412, 708.0]]))
+    for fn in (1, 10):
+        assert np.array_equal(bits(u1.selftest_math(fn, xr)), bits(orc.exp(xr)))
+    assert np.array_equal(bits(u1.selftest_math(10, xe)), bits(orc.exp(xe)))
     assert np.array_equal(bits(u1.selftest_math(2, x, y)), bits(orc.atan2(x, y)))
     xb = np.abs(x[:2000])
     b = u1.selftest_math(5, xb)

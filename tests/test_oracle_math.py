@@ -95,6 +95,25 @@ def test_exp_ulp(orc):
     assert np.isinf(big[0]) and big[1] == 0.0
 
 
+def test_exp_range_ends(orc):
+    """exp at the ends of its range, where the scaling by 2^k leaves the
+    normal numbers: subnormal results (x in [-745.14, -708.3]) within one unit
+    of the subnormal spacing of glibc's exp, results near the overflow
+    threshold within one ulp, infinity where glibc overflows, and 0, inf and
+    NaN at -inf, inf and NaN."""
+    rng = np.random.default_rng(12)
+    lo = np.concatenate([rng.uniform(-745.14, -708.3, 50000), np.linspace(-745.1332191019412, -745.13, 1001)])
+    y = orc.exp(lo)
+    ref = np.array([math.exp(v) for v in lo])
+    assert np.max(np.abs(y - ref)) <= 2.0 ** -1074
+    hi = np.concatenate([rng.uniform(700.0, 709.78, 50000), np.linspace(709.78, 709.782712893384, 1001)])
+    assert ulp_err(orc.exp(hi), [math.exp(v) for v in hi]).max() <= 1.0
+    over = np.array([709.7828, 709.79, 710.0, 1e300])
+    assert np.all(np.isinf(orc.exp(over)))
+    e = orc.exp(np.array([-np.inf, np.inf, np.nan]))
+    assert e[0] == 0.0 and np.isinf(e[1]) and np.isnan(e[2])
+
+
 def test_atan2_ulp_and_quadrants(orc):
     rng = np.random.default_rng(3)
     yv = np.concatenate([rng.uniform(-10, 10, 100000), rng.normal(size=20000) * 1e-6])
