@@ -251,12 +251,21 @@ template <typename Real>
 struct UMath;
 template <>
 struct UMath<double> {
-  static __device__ __forceinline__ void sincos(double x, double* s, double* c) { u1m::sincos_d(x, s, c); }
+  // sincos_d's bits: sincos_d_main, then its small-argument case only in a
+  // wavefront where some lane has |x| < 2^-27 (one vote instead of a test
+  // and a 64-bit select per call)
+  template <bool PIN = false>
+  static __device__ __forceinline__ void sincos_voted(double x, double* s, double* c) {
+    u1m::sincos_d_main<PIN>(x, s, c);
+    if (!__all(__builtin_fabs(x) >= 0x1p-27))
+      if (__builtin_fabs(x) < 0x1p-27) *s = x;
+  }
+  static __device__ __forceinline__ void sincos(double x, double* s, double* c) { sincos_voted<false>(x, s, c); }
   static __device__ __forceinline__ double exp(double x) { return u1m::exp_d(x); }
   // the same functions for calls inside a loop (coefficients pinned to
   // scalar registers, u1m::kc; identical results)
   static __device__ __forceinline__ void sincos_in_loop(double x, double* s, double* c) {
-    u1m::sincos_d<true>(x, s, c);
+    sincos_voted<true>(x, s, c);
   }
   // exp_d's bits for every x: exp_d_inrange, then exp_d's three special
   // cases only in a wavefront where some lane's argument is outside

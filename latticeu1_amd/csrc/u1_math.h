@@ -68,9 +68,10 @@ U1_HD bool signbitd(double x) { return __builtin_signbit(x); }
 U1_HD bool signbitf_(float x) { return __builtin_signbit(x); }
 
 // ---------------------------------------------------------------- FP64 sincos
-// PIN: see kc (the same results either way).
+// sincos_d_main: sincos_d without its case |x| < 2^-27, where sincos_d
+// returns sin x = x itself.  PIN: see kc (the same results either way).
 template <bool PIN = false>
-U1_HD void sincos_d(double x, double* sp, double* cp) {
+U1_HD void sincos_d_main(double x, double* sp, double* cp) {
   const double k = rintd(x * K_TWO_OVER_PI);
   double r = fmad(-k, K_PIO2_1, x);
   r = fmad(-k, K_PIO2_2, r);
@@ -93,11 +94,18 @@ U1_HD void sincos_d(double x, double* sp, double* cp) {
   // integer op on the high word instead of a negation and a 64-bit select
   const double sa = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) ^ ((uint64_t)(q & 2) << 62));
   const double sb = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) ^ ((uint64_t)((q + 1) & 2) << 62));
+  *sp = sa;
+  *cp = sb;
+}
+
+template <bool PIN = false>
+U1_HD void sincos_d(double x, double* sp, double* cp) {
+  double sa;
+  sincos_d_main<PIN>(x, &sa, cp);
   // |x| < 2^-27: sin x = x to binary64 precision, which also keeps sin(-0) =
   // -0 (the reduction turns -0 into +0); cos needs no such case, since there
   // k = 0, r = x, and 1 + r^2 Q(r^2) rounds to exactly 1
   *sp = absd(x) < 0x1p-27 ? x : sa;
-  *cp = sb;
 }
 
 U1_HD double sin_d(double x) { double s, c; sincos_d(x, &s, &c); return s; }
