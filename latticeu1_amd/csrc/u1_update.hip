@@ -378,29 +378,39 @@ __device__ __forceinline__ void phase_compute(Real* __restrict__ lat_base, const
     // binary32 hit constants (metro_hit_f32), rounded on the host
     const float bl2e = r.bl2e_f;
     const float pscale = a.pscale_f, poff = a.poff_f, thm = a.thm_f;
-    for (int h = 0; h < a.nhit; h += 2) {
-      const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
+    // hits in blocks of up to four (two Philox draws of four words each)
+    for (int h4 = 0; h4 < a.nhit; h4 += 4) {
 #pragma unroll
-      for (int k = 0; k < SPT; ++k) {
-        const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
-        if constexpr (sizeof(Real) == sizeof(float)) {
-          const bool ok0 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[0], w.v[1]);
-          acc_wave += count_wave(ok0, vmask[k]);
-          if (h + 1 < a.nhit) {
-            const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[2], w.v[3]);
-            acc_wave += count_wave(ok1, vmask[k]);
-          }
-          // back into [-pi, pi] after every second pair of hits and after the
-          // last (|theta| <= pi + 4 step in between)
-          if ((h & 2) || h + 2 >= a.nhit) m[k].th = (float)u1m::wrap_near_d((double)m[k].th);
-        } else {
-          const bool ok0 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
-          acc_wave += count_wave(ok0, vmask[k]);
-          if (h + 1 < a.nhit) {
-            const bool ok1 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
-            acc_wave += count_wave(ok1, vmask[k]);
+      for (int j = 0; j < 2; ++j) {
+        const int h = h4 + 2 * j;
+        if (h >= a.nhit) break;
+        const uint32_t c2 = u1r::ctr2(a.stream, a.pass, (uint32_t)(h >> 1));
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+          const u1r::U32x4 w = u1r::draw(r.seed, l[k], sweep, c2, r.c3);
+          if constexpr (sizeof(Real) == sizeof(float)) {
+            const bool ok0 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[0], w.v[1]);
+            acc_wave += count_wave(ok0, vmask[k]);
+            if (h + 1 < a.nhit) {
+              const bool ok1 = metro_hit_f32(m[k], S[k], bl2e, thm, pscale, poff, w.v[2], w.v[3]);
+              acc_wave += count_wave(ok1, vmask[k]);
+            }
+          } else {
+            const bool ok0 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[0], w.v[1]);
+            acc_wave += count_wave(ok0, vmask[k]);
+            if (h + 1 < a.nhit) {
+              const bool ok1 = metro_hit<Real>(m[k], S[k], beta, a.proposal, a.step, w.v[2], w.v[3]);
+              acc_wave += count_wave(ok1, vmask[k]);
+            }
           }
         }
+      }
+      // binary32: back into [-pi, pi] after each block, i.e. after every
+      // second pair of hits and after the last (|theta| <= pi + 4 step in
+      // between)
+      if constexpr (sizeof(Real) == sizeof(float)) {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) m[k].th = (float)u1m::wrap_near_d((double)m[k].th);
       }
     }
 #pragma unroll
