@@ -138,6 +138,23 @@ def test_fp64_sweep_parity_8x4(u1, orc, start, mode):
             say(f"  sweep {s + 1}: <P> = {u1.mean_plaquette(p_gpu):.12f}, max |dplaq| = {worst:.2e}")
 
 
+@pytest.mark.parametrize("nhit", [2, 3, 5, 6])
+def test_fp64_hit_counts_parity(u1, orc, nhit):
+    """The kernels run the hits in blocks of four (two Philox draws): hit
+    counts that end inside a block (one pair, a pair and a single hit, a
+    second block of one or two hits) give the oracle's links and accept
+    counts bit for bit, 8^4, hot start, 10 sweeps of 2 Metropolis + 1 OR."""
+    g, og, lat, ref = make_pair(u1, orc, (8, 8, 8, 8), 1.0, 1.0, True, seed=91)
+    m = u1.metro(1, nhit, 0.5)
+    om = orc.metro(1, nhit, 0.5)
+    say(f"fp64 parity nhit={nhit}")
+    for s in range(10):
+        a_gpu = lat.update(2, 1, m, 91, s, count_accepts=True)
+        a_orc = orc.update(og, ref, 2, 1, om, 91, s, count_accepts=True)
+        assert a_gpu == a_orc, f"nhit {nhit} sweep {s}: accepts {a_gpu} != {a_orc}"
+        assert np.array_equal(bits(lat.to_host()), bits(ref)), f"nhit {nhit} sweep {s}: links differ"
+
+
 @pytest.mark.parametrize("dims,beta,aniso,metrop,ovrn,nsw", [
     ((4, 4, 4, 8), 1.5, 2.0, 1, 3, 30),      # anisotropic (space-time staples weighted xi)
     ((6, 10, 4, 12), 0.9, 1.0, 2, 1, 20),    # extents that are not powers of two, 2 Metropolis passes
